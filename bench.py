@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""LolaHIP benchmark: voxel-driven cone-beam backprojection on one MI355X.
+
+A step is one full reconstruction: all 496 views of the L=512 workload
+backprojected into the 512^3 float32 volume, projections and volume resident
+in HBM when the timed region starts (the projections of a seeded 10-ellipsoid
+phantom, generated on the GPU).  Reported:
+
+  value        GUPS (L^3 * views / s, the RabbitCT convention) of the fused
+               FP32 kernel, the headline variant
+  variants     GUPS of plain FP32 (one view per launch), fused FP32, fused
+               BF16 and FP16 projection storage, with their error against the
+               plain FP32 volume
+  roofline     the headline kernel: effective voxel-view updates per launch x
+               (4*s_p + 8/b) bytes / mean launch time (HIP events on the
+               launch stream), against the 8 TB/s HBM peak
+  cpu_baseline the reference's own LolaASM (and LolaOPT) from oracle/_ref,
+               run by the reference runner on this host on a sample of views
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from rabbitct_amd import lolahip, rct  # noqa: E402
+from rabbitct_amd.generate import generate_into  # noqa: E402
+from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry  # noqa: E402
+
+METRIC = "GUPS at L=512 plain vs fused/bf16; volumes/s sparse-view+CNN at fixed RMSE"
+HBM_PEAK = 8.0e12
+SEED = 512
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_context(args, geom, phantom, ranges, kernel, prec, vpp):
+    ctx = lolahip.Context(args.L, geom.width, geom.height, geom.n_views, n_slots=geom.n_views,
+                          precision=prec, kernel=kernel, views_per_pass=vpp,
+                          run_length=args.run)
+    ctx.set_ranges(ranges)
+    generate_into(ctx, geom, phantom)
+    ctx.synchronize()
+    return ctx
+
+
+def time_variant(args, ctx, n_views):
+    """Warm-up reps, one rep whose volume is kept for the error report, then
+    the timed reps (each rep = one step = all views)."""
+    if args.warmup:
+        ctx.timed_backproject(n_views, args.warmup)
+    ctx.zero_volume()
+    ctx.timed_backproject(n_views, 1)
+    vol = ctx.get_volume()
+    ms, ms_launch, launches = ctx.timed_backproject(n_views, args.steps)
+    return ms, ms_launch, launches, vol
+
+
+def cpu_baseline(args, geom, phantom, ranges):
+    """The reference runner (oracle/_ref) on this host: LolaASM and LolaOPT on
+    a sample of the same views, with the same clip ranges."""
+    from oracle import oracle as orc
+    isa = orc.host_isa()
+    runner = orc.ref_runner(isa)
+    if runner is None:
+        return None
+    n_asm, n_opt = args.cpu_views, max(4, args.cpu_views // 4)
+    views = np.arange(n_asm)
+    with lolahip.Context(args.L, geom.width, geom.height, geom.n_views, n_slots=n_asm) as ctx:
+        imgs = generate_into(ctx, geom, phantom, views=views, want_host=True)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for alg, n in (("LolaASM", n_asm), ("LolaOPT", n_opt)):
+            hdr = rct.RctHeader(geom.width, geom.height, n, (1000.0, 0.0))
+            rct.write_rct(os.path.join(d, "in.rct"), hdr, geom.matrices[:n], imgs[:n])
+            rct.write_geometry(os.path.join(d, "geo.rct"), hdr, geom.matrices[:n])
+            rct.write_clip_file(os.path.join(d, "clip.rct"), ranges[:n])
+            env = {**os.environ, "OMP_NUM_THREADS": str(cores), "OMP_PROC_BIND": "close",
+                   "OMP_PLACES": "cores", "OMP_STACKSIZE": "64M"}
+            log(f"cpu baseline: {alg} ({isa}) on {n} views, {cores} threads")
+            p = subprocess.run([runner, "-i", "in.rct", "-a", "geo.rct", "-m", alg, "-s",
+                                str(args.L), "-b", "1", "-C", "clip.rct"], cwd=d, env=env,
+                               capture_output=True, text=True, timeout=900)
+            if p.returncode != 0:
+                log(p.stdout[-2000:] + p.stderr[-2000:])
+                return None
+            total = float(re.search(r"Total: ([0-9.eE+-]+) s", p.stdout).group(1))
+            out[alg] = {"seconds": total, "views": n,
+                        "gups": args.L ** 3 * n / total / 1e9}
+    asm = out["LolaASM"]
+    return {"value": round(asm["gups"], 3), "unit": "GUPS", "cores": cores,
+            "kind": "reference",
+            "sample": f"LolaASM ({isa}, VECTORSIZE {16 if isa == 'avx512' else 8}) over the "
+                      f"first {asm['views']} of the 496 views at L={args.L}, -b 1, reference "
+                      f"runner Total {asm['seconds']:.3f} s",
+            "lolaopt_gups": round(out["LolaOPT"]["gups"], 3),
+            "lolaopt_sample": f"first {out['LolaOPT']['views']} views, Total "
+                              f"{out['LolaOPT']['seconds']:.3f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=512)
+    ap.add_argument("--views", type=int, default=496)
+    ap.add_argument("--vpp", type=int, default=32, help="views per register pass (fused)")
+    ap.add_argument("--run", type=int, default=16, help="voxels per thread along x")
+    ap.add_argument("--variants", default="plain-f32,fused-f32,fused-bf16,fused-f16")
+    ap.add_argument("--headline", default="fused-f32")
+    ap.add_argument("--cpu-views", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the z-slab sharding is a design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    geom = ScanGeometry(n_views=args.views, seed=SEED).build()
+    phantom = EllipsoidPhantom(n_ell=10, seed=SEED).build()
+    log(f"geometry: L={args.L}, {args.views} views, {geom.width}x{geom.height}")
+    ranges = lolahip.compute_line_ranges(args.L, geom.width, geom.height, geom.matrices, 16,
+                                         device=True)
+    results, vols, lat = {}, {}, {}
+    variants = args.variants.split(",")
+    if args.headline not in variants:
+        variants.insert(0, args.headline)
+    for var in variants:
+        kernel, prec = var.split("-")
+        vpp = 1 if kernel == "plain" else args.vpp
+        t0 = time.time()
+        ctx = make_context(args, geom, phantom, ranges, kernel, prec, vpp)
+        log(f"{var}: context + {args.views} views generated in {time.time() - t0:.1f} s")
+        ms, ms_launch, launches, vol = time_variant(args, ctx, args.views)
+        eff = ctx.effective_updates(args.views)
+        ctx.close()
+        nominal = args.L ** 3 * args.views
+        results[var] = {"ms_per_step": round(ms, 3), "gups": round(nominal / ms / 1e6, 2),
+                        "gups_effective": round(eff / ms / 1e6, 2),
+                        "ms_per_launch": round(ms_launch, 4), "launches": launches,
+                        "views_per_pass": vpp}
+        lat[var] = (ms_launch, launches, eff, vpp, prec)
+        vols[var] = vol
+        log(f"{var}: {ms:.2f} ms/step  {results[var]['gups']} GUPS "
+            f"({results[var]['gups_effective']} effective)")
+    base = vols.get("plain-f32", vols[args.headline])
+    rng = float(base.max() - base.min())
+    for var, vol in vols.items():
+        d = vol - base
+        results[var]["max_err_vs_plain_f32"] = float(np.abs(d).max() / rng)
+        results[var]["rmse_vs_plain_f32"] = float(np.sqrt((d.astype(np.float64) ** 2).mean()) / rng)
+
+    ms_launch, launches, eff, vpp, prec = lat[args.headline]
+    s_p = 4 if prec == "f32" else 2
+    bytes_per_update = 4 * s_p + 8.0 / vpp
+    achieved = (eff / launches) * bytes_per_update / (ms_launch * 1e-3)
+    head = results[args.headline]
+    line = {
+        "metric": METRIC,
+        "value": head["gups"],
+        "unit": "GUPS (L^3 x views / s)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: seeded 10-ellipsoid phantom (seed {SEED}), FDK-filtered analytic "
+                "projections generated on the GPU",
+        "config": {"workload": f"backprojection L={args.L} (R_L=0.5 mm), {args.views} views of "
+                               f"{geom.width}x{geom.height} float32, line clipping VECTORSIZE 16, "
+                               f"{args.headline} b={vpp}",
+                   "L": args.L, "views": args.views, "detector": [geom.width, geom.height],
+                   "views_per_pass": vpp, "run_length": args.run},
+        "gups_effective": head["gups_effective"],
+        "variants": results,
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "bytes_per_update": bytes_per_update,
+                     "updates_per_launch": int(eff / launches)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args, geom, phantom, ranges)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

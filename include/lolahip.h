@@ -1,0 +1,215 @@
+/* lolahip.h -- C-ABI of LolaHIP, an MI355X (gfx950) cone-beam backprojector
+ * that plugs into RabbitCT's algorithm registry.
+ *
+ * Two layers, both plain C linkage with plain pointers and sizes:
+ *
+ *  1. The RabbitCT module interface: three functions of type
+ *     int (*)(RabbitCtGlobalData *), registered as
+ *     { "LolaHIP", lolaHipPrepare, lolaHipBackprojection, lolaHipFinish }.
+ *  2. The device layer under it (lh*): a reconstruction context that owns the
+ *     volume, the line ranges and a set of projection slots in HBM, for
+ *     callers that keep their inputs resident on the GPU.
+ *
+ * Reference interfaces named below are files of the RabbitCT sources.
+ */
+#ifndef LOLAHIP_H
+#define LOLAHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The module's data carrier.  Layout-identical to the reference's
+ * Projection / RabbitCtGlobalData (rabbitCt.h:9-27); when the reference header
+ * has been included first its definitions are used instead (same include
+ * guard), so a registry file can include both headers. */
+#ifndef RABBITRUNNER_TYPES_H
+#define RABBITRUNNER_TYPES_H
+typedef struct {
+  int id;         /* 1-based view number (ctFileReader.c:47-61) */
+  double *matrix; /* 12 doubles, column-major 3x4: a[c*3+r] */
+  float *image;   /* imageHeight x imageWidth floats, row-major */
+} Projection;
+
+typedef struct {
+  uint32_t problemSize;   /* L */
+  uint32_t imageWidth;    /* W */
+  uint32_t imageHeight;   /* H */
+  float voxelSize;        /* R_L, mm */
+  float O_Index;          /* O_L = -R_L*(L-1)/2, mm */
+  float *volumeData;      /* L^3 floats, x fastest: f[(z*L+y)*L+x] */
+  uint32_t numberOfProjections; /* N at prepare (with -a), b at iterate */
+  double *globalGeometry; /* N x 12 doubles (valid only with -a) */
+  Projection *projectionBuffer; /* b buffered views */
+  char *clipFilename;     /* -C file or NULL */
+} RabbitCtGlobalData;
+#endif
+
+/* One line range: voxels x in [start, stop) of a (z,y) line are updated
+ * (analyseGeometry_types.h:23-26). */
+typedef struct {
+  int16_t start;
+  int16_t stop;
+} LhLineRange;
+
+/* Bounding box of the volume's shadow on the detector
+ * (OutShadowType, analyseGeometry_types.h:16-21). */
+typedef struct {
+  int32_t Umin, Umax, Vmin, Vmax;
+} LhShadow;
+
+/* ---------------------------------------------------------------------------
+ * 1. RabbitCT module interface (moduleLoader_types.h:10-12).
+ * Errors follow the reference's convention: a message on stderr and
+ * exit(EXIT_FAILURE) (error.h:21-31, LolaOPT.c:26-31).  Return value 1.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces lolaOptPrepare (LolaOPT.c:23-84): requires the global geometry
+ * (-a), builds or loads the line ranges through the reference's clip-file
+ * contract (analyseGeometry.c:102-298), allocates the volume, the ranges and
+ * the projection slots on the GPU and uploads the (zeroed) volume. */
+int lolaHipPrepare(RabbitCtGlobalData *rcgd);
+
+/* Replaces lolaOptBackprojection (LolaOPT.c:91-179): backprojects the
+ * rcgd->numberOfProjections buffered views into the volume (+=).  Views are
+ * collected into batches of LOLAHIP_VIEWS_PER_PASS; the batch holding view
+ * id == N is finished before this call returns. */
+int lolaHipBackprojection(RabbitCtGlobalData *rcgd);
+
+/* Replaces lolaOptFinish (LolaOPT.c:86-89): flushes pending views, copies the
+ * volume into rcgd->volumeData and releases the GPU resources. */
+int lolaHipFinish(RabbitCtGlobalData *rcgd);
+
+/* ---------------------------------------------------------------------------
+ * Geometry analysis with plain pointers (analyseGeometry.h:12-13).
+ * ------------------------------------------------------------------------- */
+
+/* computeLineRanges (analyseGeometry.c:207-281), CPU form: ranges for views
+ * [0, nViews) of geometry (nViews x 12 doubles) into out[nViews*L*L],
+ * aligned to vectorSize (4, 8 or 16).  Returns 0 or a negative LH_E* code. */
+int lhComputeLineRanges(uint32_t L, float R_L, float O_L, uint32_t W, uint32_t H,
+    uint32_t nViews, const double *geometry, int vectorSize, LhLineRange *out);
+
+/* The same on the GPU (bit-identical); out is host memory. */
+int lhComputeLineRangesDevice(uint32_t L, float R_L, float O_L, uint32_t W, uint32_t H,
+    uint32_t nViews, const double *geometry, int vectorSize, LhLineRange *out);
+
+/* The clip-file contract of computeLineRanges (analyseGeometry.c:110-197,
+ * 283-296): path = clipFilename, or "RabbitInput/LineRange<L>-<VS>.rct" when
+ * NULL; an existing file of N*L*L*4 bytes is read, a missing one is computed
+ * and written.  Returns 0, or a negative LH_E* code (size mismatch, I/O). */
+int lhLineRangesFile(const char *clipFilename, uint32_t L, float R_L, float O_L,
+    uint32_t W, uint32_t H, uint32_t nViews, const double *geometry, int vectorSize,
+    LhLineRange *out);
+
+/* computeShadowOfProjection (analyseGeometry.c:50-93). */
+void lhComputeShadow(uint32_t L, float R_L, float O_L, uint32_t W, uint32_t H,
+    uint32_t nViews, const double *geometry, LhShadow *out);
+
+/* ---------------------------------------------------------------------------
+ * 2. Device layer.  No counterpart in the reference: it is the GPU form of
+ * the state LolaOPT keeps in statics (LolaOPT.c:18-21) plus the volume.
+ * ------------------------------------------------------------------------- */
+
+enum { LH_PREC_F32 = 0, LH_PREC_BF16 = 1, LH_PREC_F16 = 2 };
+enum { LH_KERNEL_PLAIN = 0, LH_KERNEL_FUSED = 1 };
+
+enum {
+  LH_OK = 0,
+  LH_E_ARG = -1,      /* invalid argument / unsupported size */
+  LH_E_NODEV = -2,    /* no visible GPU */
+  LH_E_NOMEM = -3,    /* device or pinned allocation failed */
+  LH_E_HIP = -4,      /* other HIP runtime error */
+  LH_E_IO = -5,       /* clip file read/write/size error */
+  LH_E_STATE = -6     /* call out of order */
+};
+
+typedef struct {
+  uint32_t L;          /* volume edge, multiple of 64 */
+  float R_L;           /* voxel size, mm */
+  float O_L;           /* volume origin, mm */
+  uint32_t W, H;       /* detector size, pixels */
+  uint32_t nViews;     /* N: number of views (ids 1..N) */
+  uint32_t nSlots;     /* projection slots resident in HBM */
+  int precision;       /* LH_PREC_*: storage type of the projections */
+  int kernel;          /* LH_KERNEL_*: plain launch or persistent fused */
+  uint32_t viewsPerPass; /* fused: views accumulated per register pass */
+  uint32_t runLength;  /* voxels along x per thread: 8, 16 or 32 (0 = auto) */
+  int clip;            /* 1: line ranges (LolaOPT semantics), 0: all voxels */
+  int device;          /* HIP device ordinal */
+} LhConfig;
+
+typedef struct LhContext LhContext;
+
+/* Fills cfg with defaults for (L, W, H, nViews). */
+void lhDefaultConfig(LhConfig *cfg, uint32_t L, uint32_t W, uint32_t H, uint32_t nViews);
+
+int lhCreate(const LhConfig *cfg, LhContext **out);
+int lhDestroy(LhContext *ctx);
+
+/* Line ranges of all N views (N*L*L entries, host memory); required when
+ * cfg.clip == 1. */
+int lhSetRanges(LhContext *ctx, const LhLineRange *ranges);
+/* Computes the ranges on the GPU from geometry (N x 12) instead. */
+int lhSetRangesFromGeometry(LhContext *ctx, const double *geometry, int vectorSize);
+
+/* Volume upload / zero / download (host memory, L^3 floats). */
+int lhSetVolume(LhContext *ctx, const float *host);
+int lhZeroVolume(LhContext *ctx);
+int lhGetVolume(LhContext *ctx, float *host);
+
+/* Uploads view `id` (1-based) into projection slot `slot`: the matrix
+ * (12 doubles) and the W x H float image (host memory), converted to the
+ * context's storage precision. */
+int lhLoadView(LhContext *ctx, uint32_t slot, int id, const double *matrix,
+    const float *image);
+/* Same, with the image already in device memory (W x H floats). */
+int lhLoadViewDevice(LhContext *ctx, uint32_t slot, int id, const double *matrix,
+    const float *dImage);
+
+/* Enqueues the backprojection of slots [firstSlot, firstSlot+count) on the
+ * context's stream (asynchronous). */
+int lhBackproject(LhContext *ctx, uint32_t firstSlot, uint32_t count);
+
+/* Runs `reps` times the backprojection of all slots [0, count) in passes of
+ * cfg.viewsPerPass and returns, measured with HIP events on the context's
+ * stream, the mean milliseconds per rep and per-launch statistics. */
+int lhTimedBackproject(LhContext *ctx, uint32_t count, int reps, float *msPerRep,
+    float *msPerLaunch, uint32_t *launchesPerRep);
+
+int lhSynchronize(LhContext *ctx);
+
+/* Effective voxel-view updates of views [0,count) of the slots: the number of
+ * voxels inside their line ranges (L^3 per view without clipping). */
+uint64_t lhEffectiveUpdates(LhContext *ctx, uint32_t count);
+
+/* Synthetic input (no counterpart in the reference): for views
+ * [firstId, firstId+count) fills slots [firstSlot, ...) with ramp-filtered,
+ * cosine- and redundancy-weighted analytic line integrals of an ellipsoid
+ * phantom.  Per view v: src[3v..] source position, minv[9v..] row-major
+ * inverse of the matrix's 3x3 part (ray through pixel (i,j) along
+ * minv*(i,j,1)), axis[3v..] unit principal axis, parker[W*v..] column
+ * weights; ell = nEll x 16 doubles (centre 3, semi-axes 3, density, rotation
+ * 9 row-major); tau = ramp-filter sample spacing (mm), dbeta = angular step
+ * (rad), border = zero the outermost pixel ring.  hostImages (optional,
+ * count*H*W floats) receives the float32 projections. */
+int lhGenerateViews(LhContext *ctx, uint32_t firstSlot, uint32_t count, int firstId,
+    const double *matrices, const double *src, const double *minv, const double *axis,
+    const float *parker, uint32_t nEll, const double *ell, double tau, double dbeta,
+    int border, float *hostImages);
+
+/* Last error message of the calling thread. */
+const char *lhLastError(void);
+
+/* Byte offsets of the RabbitCtGlobalData fields, for ABI checks:
+ * out[0..9] = offsetof each field in declaration order, out[10] = sizeof. */
+void lhAbiLayout(uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOLAHIP_H */

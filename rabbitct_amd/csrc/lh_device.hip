@@ -1,0 +1,878 @@
+// lh_device.hip -- LolaHIP device layer: CDNA4 (gfx950) backprojection kernels
+// and the extern "C" context API declared in include/lolahip.h.
+//
+// The path is a gather-interpolate-accumulate, not a contraction: no MFMA.
+// Work decomposition of the backprojection kernel (bp_kernel):
+//   * a workgroup is 4 waves; lane l of wave w owns the voxels
+//     (x0 .. x0+R-1, y0+l, z0+w): one x-run of R voxels per lane, 64 adjacent
+//     lines per wave, so one instruction's 64 lanes step along y and their
+//     detector taps stay within a few cache lines;
+//   * the R accumulators live in VGPRs across all views of a launch (a "pass"),
+//     so the volume is read and written once per pass, not once per view;
+//   * per view and lane, the homogeneous (u,v,w) at the run start is computed
+//     in FP64 (LolaOPT does the same per line, LolaOPT.c:140-142) and rounded
+//     to FP32; voxel r of the run then uses fma(r, a*R_L, origin) in FP32, one
+//     rounding from the origin, so nothing drifts along x;
+//   * the view's matrix and increments are wave-uniform (scalar loads); the
+//     lane's line range (int16 start, stop) is one coalesced dword per lane;
+//   * taps come from a zero-bordered copy of the projection through
+//     buffer loads (the bounds of the buffer resource turn any stray offset
+//     into a zero instead of a fault); coordinates are clamped to [-2, W] x
+//     [-2, H] so every tap lands in the 2-pixel zero border or the image.
+//
+// Interpolation semantics follow LolaOPT.c:144-173: truncation toward zero
+// (negative fractions for coordinates in (-1,0)), horizontal then vertical
+// lerp, weight 1/w^2, zero outside the detector, update only x in
+// [start, stop) of the line's range (LolaOPT.c:125-128).
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstddef>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lolahip.h"
+#include "lh_internal.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+int lh_fail(int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+namespace {
+#define fail lh_fail
+
+#define LH_HIP(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(e_ == hipErrorOutOfMemory ? LH_E_NOMEM : LH_E_HIP, "%s:%d %s: %s",      \
+          __FILE__, __LINE__, #call, hipGetErrorString(e_));                             \
+  } while (0)
+
+constexpr uint32_t kBorder   = 2;          // zero columns/rows on each side
+constexpr uint32_t kRsrcWord3 = 0x00020000; // raw buffer descriptor, gfx9 family
+constexpr int kWaves         = 4;          // waves per workgroup
+constexpr int kWave          = 64;
+
+// Per-slot view parameters, wave-uniform in the kernel (scalar loads).
+struct alignas(16) ViewParams {
+  double a[12];    // FP64 matrix, column-major a[c*3+r], times k = 2^-m
+  float du, dv, dw; // a[0..2] * R_L in FP32: per-voxel step along x
+  int id;          // 1-based view id (selects the line ranges)
+  float scale;     // storage scale s = 4^m of the projection (1 for FP32)
+  int pad[3];
+};
+static_assert(sizeof(ViewParams) == 128, "ViewParams layout");
+
+struct BpArgs {
+  float *vol;
+  const int *ranges;       // LhLineRange as packed int32: low half start, high half stop
+  const ViewParams *views;
+  const uint8_t *slots;
+  uint64_t slotBytes;      // stride between slots
+  uint32_t rowBytes;       // stride between padded rows
+  uint32_t slotRecords;    // bytes of one slot addressed by the buffer resource
+  uint32_t baseOffset;     // byte offset of pixel (0,0) relative to element (-2,-2)
+  int L;
+  float R_L, O_L;
+  float Wf, Hf;            // clamp bounds: W and H as float
+  int firstSlot, nViews;
+  int clip;
+  int tilesX, tilesY, tilesZ; // tile grid (x-runs, 64-line groups, 4-slice groups)
+  int persistent;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const BpArgs &A, int slot)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(A.slots + (uint64_t)slot * A.slotBytes), (short)0, (int)A.slotRecords,
+      (int)kRsrcWord3);
+}
+
+__device__ __forceinline__ float bf16_lo(uint32_t q) { return __uint_as_float(q << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t q) { return __uint_as_float(q & 0xffff0000u); }
+__device__ __forceinline__ float f16_lo(uint32_t q)
+{
+  return __half2float(__ushort_as_half((unsigned short)(q & 0xffffu)));
+}
+__device__ __forceinline__ float f16_hi(uint32_t q)
+{
+  return __half2float(__ushort_as_half((unsigned short)(q >> 16)));
+}
+
+// The four taps of the bilinear stencil at integer corner (iix, iiy), both
+// already clamped to [-2, W] x [-2, H].
+//  FP32: row-major padded image, two 8-byte loads (rows iiy and iiy+1).
+//  BF16/F16: "quad" layout, element (i,j) holds p(i,j), p(i+1,j), p(i,j+1),
+//  p(i+1,j+1) as 4 halves: one 8-byte load.
+template <int PREC>
+__device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, const BpArgs &A,
+    int iix, int iiy, float &p00, float &p10, float &p01, float &p11)
+{
+  if constexpr (PREC == LH_PREC_F32) {
+    const int pitch    = (int)(A.rowBytes >> 2);
+    const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
+    p00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+    p10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0));
+    p01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0));
+    p11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, (int)A.rowBytes, 0));
+  } else {
+    const int pitch    = (int)(A.rowBytes >> 3);
+    const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 8u + A.baseOffset;
+    const uint32_t q0 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+    const uint32_t q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0);
+    if constexpr (PREC == LH_PREC_BF16) {
+      p00 = bf16_lo(q0); p10 = bf16_hi(q0); p01 = bf16_lo(q1); p11 = bf16_hi(q1);
+    } else {
+      p00 = f16_lo(q0); p10 = f16_hi(q0); p01 = f16_lo(q1); p11 = f16_hi(q1);
+    }
+  }
+}
+
+// One voxel-view update: returns val / w^2 for the voxel at homogeneous
+// (u, v, w) (LolaOPT.c:145-170).
+template <int PREC>
+__device__ __forceinline__ float voxel_update(__amdgpu_buffer_rsrc_t rs, const BpArgs &A,
+    float u, float v, float w)
+{
+  const float rw = __builtin_amdgcn_rcpf(w);
+  const float ix = __builtin_amdgcn_fmed3f(u * rw, -2.0f, A.Wf);
+  const float iy = __builtin_amdgcn_fmed3f(v * rw, -2.0f, A.Hf);
+  const float fx = __builtin_truncf(ix);
+  const float fy = __builtin_truncf(iy);
+  const float ax = ix - fx;
+  const float ay = iy - fy;
+  float p00, p10, p01, p11;
+  fetch_taps<PREC>(rs, A, (int)fx, (int)fy, p00, p10, p01, p11);
+  const float b   = fmaf(ax, p10 - p00, p00);
+  const float t   = fmaf(ax, p11 - p01, p01);
+  const float val = fmaf(ay, t - b, b);
+  return val * (rw * rw);
+}
+
+template <int R, int PREC>
+__device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv   = threadIdx.x / kWave;
+  const int x0   = tx * R;
+  const int y    = ty * kWave + lane;
+  const int z    = tz * kWaves + wv;
+  const int L    = A.L;
+
+  const double wx0 = (double)A.O_L + (double)x0 * (double)A.R_L;
+  const double wy  = (double)A.O_L + (double)y * (double)A.R_L;
+  const double wz  = (double)A.O_L + (double)z * (double)A.R_L;
+  const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
+  const uint64_t LL   = (uint64_t)L * (uint64_t)L;
+
+  float acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    acc[r] = 0.0f;
+  bool touched = false;
+
+  // line range of the first view, then prefetch one view ahead
+  int rr_next = A.clip ? A.ranges[(uint64_t)(A.views[A.firstSlot].id - 1) * LL + line]
+                       : (L << 16);
+  for (int v = 0; v < A.nViews; ++v) {
+    const int slot       = A.firstSlot + v;
+    const ViewParams &P  = A.views[slot];
+    const int rr         = rr_next;
+    if (A.clip && v + 1 < A.nViews)
+      rr_next = A.ranges[(uint64_t)(A.views[slot + 1].id - 1) * LL + line];
+    const int start = (int)(short)(rr & 0xffff);
+    const int stop  = rr >> 16;
+    const int rlo   = max(start - x0, 0);
+    const int rhi   = min(stop - x0, R);
+    const bool active = rlo < rhi;
+    if (!__any(active))
+      continue;
+    touched |= active;
+
+    const float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
+    const float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
+    const float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
+    const float du = P.du, dv = P.dv, dw = P.dw;
+    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(A, slot);
+
+    if (__all(rlo == 0 && rhi == R)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float fr = (float)r;
+        acc[r] += voxel_update<PREC>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0), fmaf(fr, dw, w0));
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r >= rlo && r < rhi) {
+          const float fr = (float)r;
+          acc[r] += voxel_update<PREC>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0), fmaf(fr, dw, w0));
+        }
+      }
+    }
+  }
+
+  if (touched) {
+    float4 *p = reinterpret_cast<float4 *>(A.vol + line * (uint64_t)L + (uint64_t)x0);
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) {
+      float4 o = p[q];
+      o.x += acc[4 * q + 0];
+      o.y += acc[4 * q + 1];
+      o.z += acc[4 * q + 2];
+      o.w += acc[4 * q + 3];
+      p[q] = o;
+    }
+  }
+}
+
+// Grid form: one workgroup per tile.  Persistent form: a grid sized to the
+// chip walks the tiles; tile t goes to XCD (t mod 8) in dispatch order, so the
+// tiles are renumbered such that each XCD's share is one contiguous z-slab and
+// its L2 serves one band of detector rows.
+template <int R, int PREC>
+__global__ __launch_bounds__(kWaves *kWave) void bp_kernel(BpArgs A)
+{
+  if (!A.persistent) {
+    bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+    return;
+  }
+  const int nTiles  = A.tilesX * A.tilesY * A.tilesZ;
+  const int perXcd  = (nTiles + 7) / 8;
+  const int xcd     = blockIdx.x & 7;
+  const int local   = blockIdx.x >> 3;
+  const int stride  = (gridDim.x + 7 - xcd) / 8; // workgroups of this XCD
+  for (int k = local; k < perXcd; k += stride) {
+    const int t = xcd * perXcd + k;
+    if (t >= nTiles)
+      break;
+    const int tx = t % A.tilesX;
+    const int ty = (t / A.tilesX) % A.tilesY;
+    const int tz = t / (A.tilesX * A.tilesY);
+    bp_tile<R, PREC>(A, tx, ty, tz);
+  }
+}
+
+// ---- projection staging -------------------------------------------------
+
+// max |p| over an image, as the bit pattern of a non-negative float
+__global__ void maxabs_kernel(const float *__restrict__ img, uint32_t n, uint32_t *out)
+{
+  uint32_t m = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(fabsf(img[i])));
+  for (int o = 32; o > 0; o >>= 1)
+    m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(out, m);
+}
+
+// Storage scale s = 4^m: FP16 keeps max|p|/s in (8192, 32768]; BF16 and FP32
+// use s = 1.  A power of four keeps k = s^-1/2 a power of two, so scaling the
+// matrix by k is exact and 1/w^2 picks up exactly the factor s.
+__device__ __forceinline__ int storage_exponent(int prec, uint32_t maxbits)
+{
+  if (prec != LH_PREC_F16 || maxbits == 0)
+    return 0;
+  const float mx = __uint_as_float(maxbits);
+  int m = 0;
+  float s = 1.0f;
+  while (mx / s > 32768.0f && m < 60) { s *= 4.0f; ++m; }
+  while (mx / s <= 8192.0f && m > -60) { s *= 0.25f; --m; }
+  return m;
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+  uint32_t b = __float_as_uint(f);
+  if ((b & 0x7fffffffu) > 0x7f800000u)
+    return (uint16_t)((b >> 16) | 0x40u); // quiet NaN
+  b += 0x7fffu + ((b >> 16) & 1u);        // round to nearest even
+  return (uint16_t)(b >> 16);
+}
+
+__device__ __forceinline__ uint16_t to_f16(float f)
+{
+  return __half_as_ushort(__float2half_rn(f));
+}
+
+// Writes the bordered slot image from a W x H float image.  FP32: element
+// (i,j), i in [-2, W+1], j in [-2, H+1]; halves: quad element (i,j), i in
+// [-2, W], j in [-2, H].  Block (0,0) thread 0 also applies the storage scale
+// to the slot's view parameters.
+template <int PREC>
+__global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
+    uint32_t rowBytes, const uint32_t *maxbits, ViewParams *vp)
+{
+  const int m    = storage_exponent(PREC, *maxbits);
+  const float is = ldexpf(1.0f, -2 * m);
+  const int ei   = blockIdx.x * blockDim.x + threadIdx.x; // element column
+  const int ej   = blockIdx.y;                            // element row
+  if (PREC != LH_PREC_F32 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    const double k = ldexp(1.0, -m);
+    for (int c = 0; c < 12; ++c)
+      vp->a[c] *= k;
+    vp->du *= (float)k;
+    vp->dv *= (float)k;
+    vp->dw *= (float)k;
+    vp->scale = ldexpf(1.0f, 2 * m);
+  }
+  auto pix = [&](int i, int j) -> float {
+    return (i >= 0 && i < W && j >= 0 && j < H) ? img[(size_t)j * W + i] : 0.0f;
+  };
+  const int i = ei - (int)kBorder, j = ej - (int)kBorder;
+  if constexpr (PREC == LH_PREC_F32) {
+    if (ei >= W + 4)
+      return;
+    reinterpret_cast<float *>(slot + (size_t)ej * rowBytes)[ei] = pix(i, j);
+  } else {
+    if (ei >= W + 3)
+      return;
+    uint16_t h[4];
+    const float v[4] = { pix(i, j) * is, pix(i + 1, j) * is, pix(i, j + 1) * is,
+      pix(i + 1, j + 1) * is };
+    for (int c = 0; c < 4; ++c)
+      h[c] = PREC == LH_PREC_BF16 ? to_bf16(v[c]) : to_f16(v[c]);
+    uint2 q;
+    q.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    q.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+    reinterpret_cast<uint2 *>(slot + (size_t)ej * rowBytes)[ei] = q;
+  }
+}
+
+// ---- line ranges on the GPU ----------------------------------------------
+// Bit-identical form of analyseGeometry.c:207-281: one thread per (z,y) line
+// scans x from the left for the first hit and from the right for the last;
+// the hit test is (int)u in [0,W-1] and (int)v in [0,H-1] in FP64 on float
+// world coordinates (wz, wy accumulated by "+= mm" in float beforehand).
+__device__ __forceinline__ bool clip_hit(const double *a, float wx, float wy, float wz,
+    int W, int H)
+{
+  const double x = wx, y = wy, z = wz;
+  // the same association as the C expression a2*x + a5*y + a8*z + a11
+  const double wn = __dadd_rn(__dadd_rn(__dadd_rn(__dmul_rn(a[2], x), __dmul_rn(a[5], y)),
+                                  __dmul_rn(a[8], z)), a[11]);
+  const double un = __ddiv_rn(__dadd_rn(__dadd_rn(__dadd_rn(__dmul_rn(a[0], x),
+                        __dmul_rn(a[3], y)), __dmul_rn(a[6], z)), a[9]), wn);
+  const double vn = __ddiv_rn(__dadd_rn(__dadd_rn(__dadd_rn(__dmul_rn(a[1], x),
+                        __dmul_rn(a[4], y)), __dmul_rn(a[7], z)), a[10]), wn);
+  if (!(un > -1.0e9 && un < 1.0e9 && vn > -1.0e9 && vn < 1.0e9))
+    return false;
+  const long long iix = (long long)un, iiy = (long long)vn;
+  return iix >= 0 && iix <= W - 1 && iiy >= 0 && iiy <= H - 1;
+}
+
+__global__ void line_ranges_kernel(int L, float rL, float oL, int W, int H,
+    const double *__restrict__ geom, int vs, const float *__restrict__ wacc, int *out)
+{
+  const int view = blockIdx.y;
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= L * L)
+    return;
+  const int z = line / L, y = line % L;
+  double a[12];
+  for (int c = 0; c < 12; ++c)
+    a[c] = geom[view * 12 + c];
+  const float wz = wacc[z], wy = wacc[y];
+  int xstart = -1, xstop = -1;
+  for (int x = 0; x < L; ++x) {
+    const float wx = __fadd_rn(__fmul_rn((float)x, rL), oL);
+    if (clip_hit(a, wx, wy, wz, W, H)) { xstart = x; break; }
+  }
+  int packed = 0;
+  if (xstart >= 0) {
+    for (int x = L - 1; x >= 0; --x) {
+      const float wx = __fadd_rn(__fmul_rn((float)x, rL), oL);
+      if (clip_hit(a, wx, wy, wz, W, H)) { xstop = x + 1; break; }
+    }
+    int al;
+    if ((al = xstop % vs))
+      xstop += vs - al;
+    if ((al = xstart % vs))
+      xstart -= al;
+    packed = (int)(uint16_t)(int16_t)xstart | (int)((uint32_t)(uint16_t)(int16_t)xstop << 16);
+  }
+  out[(size_t)view * L * L + line] = packed;
+}
+
+} // namespace
+
+// ---- context ---------------------------------------------------------------
+
+struct LhContext {
+  LhConfig cfg{};
+  hipStream_t stream = nullptr;
+  float *dVol        = nullptr;
+  int *dRanges       = nullptr;
+  ViewParams *dViews = nullptr;
+  uint8_t *dSlots    = nullptr;
+  float *dStage      = nullptr;     // W*H floats
+  uint32_t *dMax     = nullptr;     // one per slot
+  float *hPinned     = nullptr;     // W*H floats, pinned
+  uint64_t slotBytes = 0;
+  uint32_t rowBytes  = 0;
+  uint32_t slotRecords = 0;
+  uint32_t baseOffset  = 0;
+  int numCU = 0;
+  std::vector<uint64_t> effPerView; // by id-1
+  std::vector<int> slotId;          // id held by each slot (0 = empty)
+};
+
+const LhConfig &lh_config(const LhContext *c) { return c->cfg; }
+hipStream_t lh_stream(const LhContext *c) { return c->stream; }
+
+extern "C" {
+
+const char *lhLastError(void) { return g_err.c_str(); }
+
+void lhAbiLayout(uint64_t *out)
+{
+  out[0]  = offsetof(RabbitCtGlobalData, problemSize);
+  out[1]  = offsetof(RabbitCtGlobalData, imageWidth);
+  out[2]  = offsetof(RabbitCtGlobalData, imageHeight);
+  out[3]  = offsetof(RabbitCtGlobalData, voxelSize);
+  out[4]  = offsetof(RabbitCtGlobalData, O_Index);
+  out[5]  = offsetof(RabbitCtGlobalData, volumeData);
+  out[6]  = offsetof(RabbitCtGlobalData, numberOfProjections);
+  out[7]  = offsetof(RabbitCtGlobalData, globalGeometry);
+  out[8]  = offsetof(RabbitCtGlobalData, projectionBuffer);
+  out[9]  = offsetof(RabbitCtGlobalData, clipFilename);
+  out[10] = sizeof(RabbitCtGlobalData);
+}
+
+void lhDefaultConfig(LhConfig *cfg, uint32_t L, uint32_t W, uint32_t H, uint32_t nViews)
+{
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->L            = L;
+  cfg->R_L          = L ? 256.0f / (float)L : 0.0f;
+  cfg->O_L          = -0.5f * cfg->R_L * ((float)L - 1.0f);
+  cfg->W            = W;
+  cfg->H            = H;
+  cfg->nViews       = nViews;
+  cfg->viewsPerPass = 32;
+  cfg->nSlots       = 2 * cfg->viewsPerPass;
+  cfg->precision    = LH_PREC_F32;
+  cfg->kernel       = LH_KERNEL_FUSED;
+  cfg->runLength    = 16;
+  cfg->clip         = 1;
+  cfg->device       = 0;
+}
+
+int lhCreate(const LhConfig *cfg, LhContext **out)
+{
+  *out = nullptr;
+  if (!cfg || cfg->L == 0 || cfg->L % 64 || cfg->L > 4096 || cfg->W < 2 || cfg->H < 2 ||
+      cfg->W > 16384 || cfg->H > 16384 || cfg->nViews == 0 || cfg->nSlots == 0)
+    return fail(LH_E_ARG, "lhCreate: unsupported sizes (L must be a multiple of 64)");
+  if (cfg->precision < LH_PREC_F32 || cfg->precision > LH_PREC_F16)
+    return fail(LH_E_ARG, "lhCreate: bad precision %d", cfg->precision);
+  const uint32_t R = cfg->runLength ? cfg->runLength : 16;
+  if ((R != 8 && R != 16 && R != 32) || cfg->L % R)
+    return fail(LH_E_ARG, "lhCreate: runLength must be 8, 16 or 32 and divide L");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(LH_E_NODEV, "no HIP device visible");
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(LH_E_ARG, "device %d out of range (%d visible)", cfg->device, ndev);
+
+  LhContext *c = new (std::nothrow) LhContext;
+  if (!c)
+    return fail(LH_E_NOMEM, "host allocation failed");
+  c->cfg           = *cfg;
+  c->cfg.runLength = R;
+  *out             = c;
+  LH_HIP(hipSetDevice(cfg->device));
+  LH_HIP(hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, cfg->device));
+  LH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+  const uint64_t L = cfg->L;
+  LH_HIP(hipMalloc(&c->dVol, L * L * L * sizeof(float)));
+  LH_HIP(hipMemsetAsync(c->dVol, 0, L * L * L * sizeof(float), c->stream));
+  if (cfg->clip)
+    LH_HIP(hipMalloc(&c->dRanges, (uint64_t)cfg->nViews * L * L * sizeof(int)));
+  LH_HIP(hipMalloc(&c->dViews, cfg->nSlots * sizeof(ViewParams)));
+  LH_HIP(hipMemsetAsync(c->dViews, 0, cfg->nSlots * sizeof(ViewParams), c->stream));
+
+  // slot layout: FP32 rows of W+4 floats, H+4 rows; halves: rows of W+3 quads
+  // (8 bytes), H+3 rows.  Rows padded to 256 bytes.
+  const uint64_t elem  = cfg->precision == LH_PREC_F32 ? 4 : 8;
+  const uint64_t cols  = cfg->precision == LH_PREC_F32 ? cfg->W + 4 : cfg->W + 3;
+  const uint64_t rows  = cfg->precision == LH_PREC_F32 ? cfg->H + 4 : cfg->H + 3;
+  c->rowBytes          = (uint32_t)((cols * elem + 255) / 256 * 256);
+  c->slotRecords       = (uint32_t)(c->rowBytes * rows);
+  c->slotBytes         = ((uint64_t)c->slotRecords + 4095) / 4096 * 4096;
+  c->baseOffset        = (uint32_t)(kBorder * c->rowBytes + kBorder * elem);
+  LH_HIP(hipMalloc(&c->dSlots, c->slotBytes * cfg->nSlots));
+  LH_HIP(hipMemsetAsync(c->dSlots, 0, c->slotBytes * cfg->nSlots, c->stream));
+  LH_HIP(hipMalloc(&c->dStage, (uint64_t)cfg->W * cfg->H * sizeof(float)));
+  LH_HIP(hipMalloc(&c->dMax, cfg->nSlots * sizeof(uint32_t)));
+  LH_HIP(hipHostMalloc((void **)&c->hPinned, (uint64_t)cfg->W * cfg->H * sizeof(float),
+      hipHostMallocDefault));
+  c->effPerView.assign(cfg->nViews, cfg->clip ? 0 : L * L * L);
+  c->slotId.assign(cfg->nSlots, 0);
+  LH_HIP(hipStreamSynchronize(c->stream));
+  return LH_OK;
+}
+
+int lhDestroy(LhContext *c)
+{
+  if (!c)
+    return LH_OK;
+  if (c->stream)
+    (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->dVol);
+  (void)hipFree(c->dRanges);
+  (void)hipFree(c->dViews);
+  (void)hipFree(c->dSlots);
+  (void)hipFree(c->dStage);
+  (void)hipFree(c->dMax);
+  if (c->hPinned)
+    (void)hipHostFree(c->hPinned);
+  if (c->stream)
+    (void)hipStreamDestroy(c->stream);
+  delete c;
+  return LH_OK;
+}
+
+static void count_effective(LhContext *c, const int *packed)
+{
+  const uint64_t LL = (uint64_t)c->cfg.L * c->cfg.L;
+  for (uint32_t n = 0; n < c->cfg.nViews; ++n) {
+    uint64_t s = 0;
+    for (uint64_t i = 0; i < LL; ++i) {
+      const int v     = packed[n * LL + i];
+      const int start = std::max((int)(short)(v & 0xffff), 0);
+      const int stop  = std::min(v >> 16, (int)c->cfg.L);
+      if (stop > start)
+        s += (uint64_t)(stop - start);
+    }
+    c->effPerView[n] = s;
+  }
+}
+
+int lhSetRanges(LhContext *c, const LhLineRange *ranges)
+{
+  if (!c || !c->cfg.clip || !ranges)
+    return fail(LH_E_STATE, "lhSetRanges: context has no clipping");
+  const uint64_t n = (uint64_t)c->cfg.nViews * c->cfg.L * c->cfg.L;
+  static_assert(sizeof(LhLineRange) == sizeof(int), "packed range");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  LH_HIP(hipMemcpyAsync(c->dRanges, ranges, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  LH_HIP(hipStreamSynchronize(c->stream));
+  count_effective(c, reinterpret_cast<const int *>(ranges));
+  return LH_OK;
+}
+
+int lhSetRangesFromGeometry(LhContext *c, const double *geometry, int vs)
+{
+  if (!c || !c->cfg.clip || !geometry || (vs != 4 && vs != 8 && vs != 16))
+    return fail(LH_E_ARG, "lhSetRangesFromGeometry: bad arguments");
+  const LhConfig &g = c->cfg;
+  std::vector<float> wacc(g.L);
+  float acc = g.O_L;
+  for (uint32_t i = 0; i < g.L; ++i, acc += g.R_L)
+    wacc[i] = acc;
+  double *dGeom = nullptr;
+  float *dW     = nullptr;
+  LH_HIP(hipSetDevice(g.device));
+  LH_HIP(hipMalloc(&dGeom, (size_t)g.nViews * 12 * sizeof(double)));
+  LH_HIP(hipMalloc(&dW, g.L * sizeof(float)));
+  LH_HIP(hipMemcpyAsync(dGeom, geometry, (size_t)g.nViews * 12 * sizeof(double),
+      hipMemcpyHostToDevice, c->stream));
+  LH_HIP(hipMemcpyAsync(dW, wacc.data(), g.L * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  dim3 grid((g.L * g.L + 255) / 256, g.nViews);
+  line_ranges_kernel<<<grid, 256, 0, c->stream>>>((int)g.L, g.R_L, g.O_L, (int)g.W, (int)g.H,
+      dGeom, vs, dW, c->dRanges);
+  LH_HIP(hipGetLastError());
+  std::vector<int> host((size_t)g.nViews * g.L * g.L);
+  LH_HIP(hipMemcpyAsync(host.data(), c->dRanges, host.size() * sizeof(int),
+      hipMemcpyDeviceToHost, c->stream));
+  LH_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(dGeom);
+  (void)hipFree(dW);
+  count_effective(c, host.data());
+  return LH_OK;
+}
+
+int lhComputeLineRangesDevice(uint32_t L, float R_L, float O_L, uint32_t W, uint32_t H,
+    uint32_t nViews, const double *geometry, int vectorSize, LhLineRange *out)
+{
+  LhConfig cfg;
+  lhDefaultConfig(&cfg, L, W, H, nViews);
+  cfg.R_L    = R_L;
+  cfg.O_L    = O_L;
+  cfg.nSlots = 1;
+  LhContext *c = nullptr;
+  int rc       = lhCreate(&cfg, &c);
+  if (rc == LH_OK)
+    rc = lhSetRangesFromGeometry(c, geometry, vectorSize);
+  if (rc == LH_OK) {
+    hipError_t e = hipMemcpy(out, c->dRanges, (size_t)nViews * L * L * sizeof(int),
+        hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+      rc = fail(LH_E_HIP, "copy of ranges: %s", hipGetErrorString(e));
+  }
+  lhDestroy(c);
+  return rc;
+}
+
+int lhSetVolume(LhContext *c, const float *host)
+{
+  if (!c || !host)
+    return fail(LH_E_ARG, "lhSetVolume: null argument");
+  const uint64_t L = c->cfg.L;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  LH_HIP(hipMemcpyAsync(c->dVol, host, L * L * L * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  LH_HIP(hipStreamSynchronize(c->stream));
+  return LH_OK;
+}
+
+int lhZeroVolume(LhContext *c)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhZeroVolume: null context");
+  const uint64_t L = c->cfg.L;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  LH_HIP(hipMemsetAsync(c->dVol, 0, L * L * L * sizeof(float), c->stream));
+  return LH_OK;
+}
+
+int lhGetVolume(LhContext *c, float *host)
+{
+  if (!c || !host)
+    return fail(LH_E_ARG, "lhGetVolume: null argument");
+  const uint64_t L = c->cfg.L;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  LH_HIP(hipMemcpyAsync(host, c->dVol, L * L * L * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  LH_HIP(hipStreamSynchronize(c->stream));
+  return LH_OK;
+}
+
+static int load_view_common(LhContext *c, uint32_t slot, int id, const double *matrix)
+{
+  const LhConfig &g = c->cfg;
+  if (slot >= g.nSlots || id < 1 || (uint32_t)id > g.nViews || !matrix)
+    return fail(LH_E_ARG, "lhLoadView: slot %u / id %d out of range", slot, id);
+  ViewParams vp;
+  std::memset(&vp, 0, sizeof vp);
+  for (int k = 0; k < 12; ++k)
+    vp.a[k] = matrix[k];
+  vp.du    = (float)(matrix[0] * (double)g.R_L);
+  vp.dv    = (float)(matrix[1] * (double)g.R_L);
+  vp.dw    = (float)(matrix[2] * (double)g.R_L);
+  vp.id    = id;
+  vp.scale = 1.0f;
+  LH_HIP(hipMemcpyAsync(c->dViews + slot, &vp, sizeof vp, hipMemcpyHostToDevice, c->stream));
+  LH_HIP(hipMemsetAsync(c->dMax + slot, 0, sizeof(uint32_t), c->stream));
+  const uint32_t n = g.W * g.H;
+  maxabs_kernel<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, c->stream>>>(c->dStage, n,
+      c->dMax + slot);
+  const int cols = g.precision == LH_PREC_F32 ? (int)g.W + 4 : (int)g.W + 3;
+  const int rows = g.precision == LH_PREC_F32 ? (int)g.H + 4 : (int)g.H + 3;
+  dim3 grid((cols + 255) / 256, rows);
+  uint8_t *dst = c->dSlots + (uint64_t)slot * c->slotBytes;
+  switch (g.precision) {
+  case LH_PREC_F32:
+    stage_kernel<LH_PREC_F32><<<grid, 256, 0, c->stream>>>(c->dStage, g.W, g.H, dst, c->rowBytes,
+        c->dMax + slot, c->dViews + slot);
+    break;
+  case LH_PREC_BF16:
+    stage_kernel<LH_PREC_BF16><<<grid, 256, 0, c->stream>>>(c->dStage, g.W, g.H, dst, c->rowBytes,
+        c->dMax + slot, c->dViews + slot);
+    break;
+  default:
+    stage_kernel<LH_PREC_F16><<<grid, 256, 0, c->stream>>>(c->dStage, g.W, g.H, dst, c->rowBytes,
+        c->dMax + slot, c->dViews + slot);
+    break;
+  }
+  LH_HIP(hipGetLastError());
+  c->slotId[slot] = id;
+  return LH_OK;
+}
+
+int lhLoadView(LhContext *c, uint32_t slot, int id, const double *matrix, const float *image)
+{
+  if (!c || !image)
+    return fail(LH_E_ARG, "lhLoadView: null argument");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
+  // the pinned staging buffer is reused: wait for the previous upload first
+  LH_HIP(hipStreamSynchronize(c->stream));
+  std::memcpy(c->hPinned, image, bytes);
+  LH_HIP(hipMemcpyAsync(c->dStage, c->hPinned, bytes, hipMemcpyHostToDevice, c->stream));
+  return load_view_common(c, slot, id, matrix);
+}
+
+int lhLoadViewDevice(LhContext *c, uint32_t slot, int id, const double *matrix,
+    const float *dImage)
+{
+  if (!c || !dImage)
+    return fail(LH_E_ARG, "lhLoadViewDevice: null argument");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
+  LH_HIP(hipMemcpyAsync(c->dStage, dImage, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return load_view_common(c, slot, id, matrix);
+}
+
+static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
+{
+  const LhConfig &g = c->cfg;
+  BpArgs A;
+  A.vol         = c->dVol;
+  A.ranges      = c->dRanges;
+  A.views       = c->dViews;
+  A.slots       = c->dSlots;
+  A.slotBytes   = c->slotBytes;
+  A.rowBytes    = c->rowBytes;
+  A.slotRecords = c->slotRecords;
+  A.baseOffset  = c->baseOffset;
+  A.L           = (int)g.L;
+  A.R_L         = g.R_L;
+  A.O_L         = g.O_L;
+  A.Wf          = (float)g.W;
+  A.Hf          = (float)g.H;
+  A.firstSlot   = (int)first;
+  A.nViews      = (int)count;
+  A.clip        = g.clip;
+  const int R   = (int)g.runLength;
+  A.tilesX      = (int)g.L / R;
+  A.tilesY      = (int)g.L / kWave;
+  A.tilesZ      = (int)g.L / kWaves;
+  A.persistent  = g.kernel == LH_KERNEL_FUSED;
+  dim3 grid;
+  if (A.persistent)
+    grid = dim3(std::min<long>((long)c->numCU * 8, (long)A.tilesX * A.tilesY * A.tilesZ));
+  else
+    grid = dim3(A.tilesX, A.tilesY, A.tilesZ);
+  dim3 block(kWaves * kWave);
+
+#define LH_LAUNCH(RR, PP) bp_kernel<RR, PP><<<grid, block, 0, c->stream>>>(A)
+#define LH_BY_PREC(RR)                                                                   \
+  switch (g.precision) {                                                                 \
+  case LH_PREC_F32: LH_LAUNCH(RR, LH_PREC_F32); break;                                    \
+  case LH_PREC_BF16: LH_LAUNCH(RR, LH_PREC_BF16); break;                                  \
+  default: LH_LAUNCH(RR, LH_PREC_F16); break;                                             \
+  }
+  switch (R) {
+  case 8: LH_BY_PREC(8); break;
+  case 32: LH_BY_PREC(32); break;
+  default: LH_BY_PREC(16); break;
+  }
+#undef LH_BY_PREC
+#undef LH_LAUNCH
+  LH_HIP(hipGetLastError());
+  return LH_OK;
+}
+
+int lhBackproject(LhContext *c, uint32_t first, uint32_t count)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhBackproject: null context");
+  const LhConfig &g = c->cfg;
+  if (count == 0)
+    return LH_OK;
+  if (first + count > g.nSlots)
+    return fail(LH_E_ARG, "lhBackproject: slots [%u,%u) beyond %u", first, first + count, g.nSlots);
+  for (uint32_t s = first; s < first + count; ++s)
+    if (c->slotId[s] == 0)
+      return fail(LH_E_STATE, "lhBackproject: slot %u holds no view", s);
+  LH_HIP(hipSetDevice(g.device));
+  const uint32_t per = g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 1u;
+  for (uint32_t s = first; s < first + count; s += per) {
+    const int rc = launch_bp(c, s, std::min(per, first + count - s));
+    if (rc != LH_OK)
+      return rc;
+  }
+  return LH_OK;
+}
+
+int lhTimedBackproject(LhContext *c, uint32_t count, int reps, float *msPerRep,
+    float *msPerLaunch, uint32_t *launchesPerRep)
+{
+  if (!c || reps < 1 || count == 0 || count > c->cfg.nSlots)
+    return fail(LH_E_ARG, "lhTimedBackproject: bad arguments");
+  const LhConfig &g = c->cfg;
+  for (uint32_t s = 0; s < count; ++s)
+    if (c->slotId[s] == 0)
+      return fail(LH_E_STATE, "lhTimedBackproject: slot %u holds no view", s);
+  const uint32_t per = g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 1u;
+  const uint32_t nl  = (count + per - 1) / per;
+  LH_HIP(hipSetDevice(g.device));
+  std::vector<hipEvent_t> ev(nl + 1);
+  for (auto &e : ev)
+    LH_HIP(hipEventCreate(&e));
+  double total = 0.0, launchSum = 0.0;
+  for (int rep = 0; rep < reps; ++rep) {
+    LH_HIP(hipEventRecord(ev[0], c->stream));
+    for (uint32_t k = 0; k < nl; ++k) {
+      const uint32_t s = k * per;
+      const int rc     = launch_bp(c, s, std::min(per, count - s));
+      if (rc != LH_OK)
+        return rc;
+      LH_HIP(hipEventRecord(ev[k + 1], c->stream));
+    }
+    LH_HIP(hipEventSynchronize(ev[nl]));
+    float ms = 0.0f;
+    LH_HIP(hipEventElapsedTime(&ms, ev[0], ev[nl]));
+    total += ms;
+    for (uint32_t k = 0; k < nl; ++k) {
+      float m = 0.0f;
+      LH_HIP(hipEventElapsedTime(&m, ev[k], ev[k + 1]));
+      launchSum += m;
+    }
+  }
+  for (auto &e : ev)
+    (void)hipEventDestroy(e);
+  if (msPerRep)
+    *msPerRep = (float)(total / reps);
+  if (msPerLaunch)
+    *msPerLaunch = (float)(launchSum / ((double)reps * nl));
+  if (launchesPerRep)
+    *launchesPerRep = nl;
+  return LH_OK;
+}
+
+int lhSynchronize(LhContext *c)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhSynchronize: null context");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  LH_HIP(hipStreamSynchronize(c->stream));
+  return LH_OK;
+}
+
+uint64_t lhEffectiveUpdates(LhContext *c, uint32_t count)
+{
+  if (!c)
+    return 0;
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < count && k < c->cfg.nSlots; ++k) {
+    const int id = c->slotId[k];
+    if (id >= 1)
+      s += c->effPerView[id - 1];
+  }
+  return s;
+}
+
+} // extern "C"

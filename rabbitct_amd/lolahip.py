@@ -1,0 +1,330 @@
+"""Python mirror of RabbitCT's module interface over liblolahip.so.
+
+* ``RabbitCtGlobalData`` / ``Projection``: ctypes views of the reference's
+  data carrier (rabbitCt.h:9-27);
+* ``ALGORITHMS`` / ``algorithm_registry_find``: the registry row
+  ``{"LolaHIP", prepare, iterate, finish}`` (algorithmRegistry.c:39-63);
+* ``run_module``: the runner's buffering loop (rabbitRunner.c:256-287) driving
+  a module through those three entry points;
+* ``Context``: the device layer (lh* functions) with numpy arguments.
+
+The library is required: every entry point raises if liblolahip.so is
+missing; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import rct
+from .phantom import voxel_size, volume_origin
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblolahip.so")
+
+PREC = {"f32": 0, "bf16": 1, "f16": 2}
+KERNEL = {"plain": 0, "fused": 1}
+
+
+class Projection(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int),
+                ("matrix", ctypes.POINTER(ctypes.c_double)),
+                ("image", ctypes.POINTER(ctypes.c_float))]
+
+
+class RabbitCtGlobalData(ctypes.Structure):
+    _fields_ = [("problemSize", ctypes.c_uint32),
+                ("imageWidth", ctypes.c_uint32),
+                ("imageHeight", ctypes.c_uint32),
+                ("voxelSize", ctypes.c_float),
+                ("O_Index", ctypes.c_float),
+                ("volumeData", ctypes.POINTER(ctypes.c_float)),
+                ("numberOfProjections", ctypes.c_uint32),
+                ("globalGeometry", ctypes.POINTER(ctypes.c_double)),
+                ("projectionBuffer", ctypes.POINTER(Projection)),
+                ("clipFilename", ctypes.c_char_p)]
+
+
+class LhLineRange(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_int16), ("stop", ctypes.c_int16)]
+
+
+class LhShadow(ctypes.Structure):
+    _fields_ = [("Umin", ctypes.c_int32), ("Umax", ctypes.c_int32),
+                ("Vmin", ctypes.c_int32), ("Vmax", ctypes.c_int32)]
+
+
+class LhConfig(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint32), ("R_L", ctypes.c_float), ("O_L", ctypes.c_float),
+                ("W", ctypes.c_uint32), ("H", ctypes.c_uint32), ("nViews", ctypes.c_uint32),
+                ("nSlots", ctypes.c_uint32), ("precision", ctypes.c_int),
+                ("kernel", ctypes.c_int), ("viewsPerPass", ctypes.c_uint32),
+                ("runLength", ctypes.c_uint32), ("clip", ctypes.c_int),
+                ("device", ctypes.c_int)]
+
+
+# Every symbol include/lolahip.h declares.
+EXPORTS = (
+    "lolaHipPrepare", "lolaHipBackprojection", "lolaHipFinish",
+    "lhComputeLineRanges", "lhComputeLineRangesDevice", "lhLineRangesFile",
+    "lhComputeShadow", "lhDefaultConfig", "lhCreate", "lhDestroy", "lhSetRanges",
+    "lhSetRangesFromGeometry", "lhSetVolume", "lhZeroVolume", "lhGetVolume",
+    "lhLoadView", "lhLoadViewDevice", "lhBackproject", "lhTimedBackproject",
+    "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
+)
+
+_lib = None
+
+
+class LolaHipError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LolaHipError(f"{path} is not built: run __graft_entry__.build() "
+                           "(there is no CPU fallback for LolaHIP)")
+    lib = ctypes.CDLL(path)
+    P, u32, f32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int
+    GD = ctypes.POINTER(RabbitCtGlobalData)
+    for name in ("lolaHipPrepare", "lolaHipBackprojection", "lolaHipFinish"):
+        getattr(lib, name).argtypes = [GD]
+        getattr(lib, name).restype = i32
+    lib.lhComputeLineRanges.argtypes = [u32, f32, f32, u32, u32, u32, P, i32, P]
+    lib.lhComputeLineRangesDevice.argtypes = [u32, f32, f32, u32, u32, u32, P, i32, P]
+    lib.lhLineRangesFile.argtypes = [ctypes.c_char_p, u32, f32, f32, u32, u32, u32, P, i32, P]
+    lib.lhComputeShadow.argtypes = [u32, f32, f32, u32, u32, u32, P, ctypes.POINTER(LhShadow)]
+    lib.lhComputeShadow.restype = None
+    lib.lhDefaultConfig.argtypes = [ctypes.POINTER(LhConfig), u32, u32, u32, u32]
+    lib.lhDefaultConfig.restype = None
+    lib.lhCreate.argtypes = [ctypes.POINTER(LhConfig), ctypes.POINTER(P)]
+    lib.lhDestroy.argtypes = [P]
+    lib.lhSetRanges.argtypes = [P, P]
+    lib.lhSetRangesFromGeometry.argtypes = [P, P, i32]
+    lib.lhSetVolume.argtypes = [P, P]
+    lib.lhZeroVolume.argtypes = [P]
+    lib.lhGetVolume.argtypes = [P, P]
+    lib.lhLoadView.argtypes = [P, u32, i32, P, P]
+    lib.lhLoadViewDevice.argtypes = [P, u32, i32, P, P]
+    lib.lhBackproject.argtypes = [P, u32, u32]
+    lib.lhTimedBackproject.argtypes = [P, u32, i32, ctypes.POINTER(f32), ctypes.POINTER(f32),
+                                       ctypes.POINTER(u32)]
+    lib.lhSynchronize.argtypes = [P]
+    lib.lhEffectiveUpdates.argtypes = [P, u32]
+    lib.lhEffectiveUpdates.restype = ctypes.c_uint64
+    lib.lhLastError.restype = ctypes.c_char_p
+    lib.lhGenerateViews.argtypes = [P, u32, u32, i32, P, P, P, P, P, u32, P, ctypes.c_double,
+                                    ctypes.c_double, i32, P]
+    lib.lhAbiLayout.argtypes = [P]
+    lib.lhAbiLayout.restype = None
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray) -> ctypes.c_void_p:
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load_library().lhLastError().decode(errors="replace")
+        raise LolaHipError(f"{what} failed ({rc}): {msg}")
+
+
+# ---- registry mirror ------------------------------------------------------
+
+def algorithm_registry_find(name: str):
+    """(prepare, iterate, finish) of a registered algorithm
+    (algorithmRegistry.c:50-63); only LolaHIP lives in this library."""
+    if name != "LolaHIP":
+        raise KeyError(f"Unknown algorithm: {name}")
+    lib = load_library()
+    return lib.lolaHipPrepare, lib.lolaHipBackprojection, lib.lolaHipFinish
+
+
+ALGORITHMS = ("LolaHIP",)
+
+
+# ---- geometry analysis ----------------------------------------------------
+
+def compute_line_ranges(L, W, H, matrices, vector_size=16, device=False, rL=None, oL=None):
+    """(N, L*L, 2) int16 ranges via lhComputeLineRanges[Device]."""
+    lib = load_library()
+    m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1, 12)
+    N = m.shape[0]
+    rL = voxel_size(L) if rL is None else rL
+    oL = volume_origin(L) if oL is None else oL
+    out = np.zeros((N, L * L, 2), dtype=np.int16)
+    fn = lib.lhComputeLineRangesDevice if device else lib.lhComputeLineRanges
+    _check(fn(L, rL, oL, W, H, N, _p(m), vector_size, _p(out)), fn.__name__)
+    return out
+
+
+def compute_shadow(L, W, H, matrices):
+    lib = load_library()
+    m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1, 12)
+    s = LhShadow()
+    lib.lhComputeShadow(L, voxel_size(L), volume_origin(L), W, H, m.shape[0], _p(m),
+                        ctypes.byref(s))
+    return {"Umin": s.Umin, "Umax": s.Umax, "Vmin": s.Vmin, "Vmax": s.Vmax}
+
+
+def abi_layout():
+    out = np.zeros(11, dtype=np.uint64)
+    load_library().lhAbiLayout(_p(out))
+    return [int(v) for v in out]
+
+
+# ---- device layer ---------------------------------------------------------
+
+class Context:
+    """A reconstruction context: volume, line ranges and projection slots
+    resident in HBM."""
+
+    def __init__(self, L, W, H, n_views, n_slots=None, precision="f32", kernel="fused",
+                 views_per_pass=32, run_length=16, clip=True, device=0, rL=None, oL=None):
+        lib = load_library()
+        cfg = LhConfig()
+        lib.lhDefaultConfig(ctypes.byref(cfg), L, W, H, n_views)
+        cfg.R_L = voxel_size(L) if rL is None else rL
+        cfg.O_L = volume_origin(L) if oL is None else oL
+        cfg.nSlots = n_views if n_slots is None else n_slots
+        cfg.precision = PREC[precision]
+        cfg.kernel = KERNEL[kernel]
+        cfg.viewsPerPass = views_per_pass
+        cfg.runLength = run_length
+        cfg.clip = 1 if clip else 0
+        cfg.device = device
+        self.cfg = cfg
+        self._h = ctypes.c_void_p()
+        _check(lib.lhCreate(ctypes.byref(cfg), ctypes.byref(self._h)), "lhCreate")
+        self.lib = lib
+
+    def close(self):
+        if self._h:
+            self.lib.lhDestroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ranges(self, ranges):
+        r = np.ascontiguousarray(ranges, dtype=np.int16)
+        _check(self.lib.lhSetRanges(self._h, _p(r)), "lhSetRanges")
+
+    def set_ranges_from_geometry(self, matrices, vector_size=16):
+        m = np.ascontiguousarray(matrices, dtype=np.float64)
+        _check(self.lib.lhSetRangesFromGeometry(self._h, _p(m), vector_size),
+               "lhSetRangesFromGeometry")
+
+    def set_volume(self, vol):
+        v = np.ascontiguousarray(vol, dtype=np.float32)
+        _check(self.lib.lhSetVolume(self._h, _p(v)), "lhSetVolume")
+
+    def zero_volume(self):
+        _check(self.lib.lhZeroVolume(self._h), "lhZeroVolume")
+
+    def get_volume(self):
+        L = self.cfg.L
+        out = np.empty((L, L, L), dtype=np.float32)
+        _check(self.lib.lhGetVolume(self._h, _p(out)), "lhGetVolume")
+        return out
+
+    def load_view(self, slot, view_id, matrix, image):
+        m = np.ascontiguousarray(matrix, dtype=np.float64)
+        img = np.ascontiguousarray(image, dtype=np.float32)
+        _check(self.lib.lhLoadView(self._h, slot, view_id, _p(m), _p(img)), "lhLoadView")
+
+    def backproject(self, first, count):
+        _check(self.lib.lhBackproject(self._h, first, count), "lhBackproject")
+
+    def synchronize(self):
+        _check(self.lib.lhSynchronize(self._h), "lhSynchronize")
+
+    def timed_backproject(self, count, reps):
+        ms, msl, nl = ctypes.c_float(), ctypes.c_float(), ctypes.c_uint32()
+        _check(self.lib.lhTimedBackproject(self._h, count, reps, ctypes.byref(ms),
+                                           ctypes.byref(msl), ctypes.byref(nl)),
+               "lhTimedBackproject")
+        return ms.value, msl.value, nl.value
+
+    def effective_updates(self, count):
+        return int(self.lib.lhEffectiveUpdates(self._h, count))
+
+
+def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel="fused",
+                      views_per_pass=32, run_length=16, volume=None):
+    """Backproject views (N,12)/(N,H,W) on the GPU; returns the (L,L,L) float32
+    volume (+= into ``volume`` when given).  ``ranges`` None = unclipped."""
+    images = np.asarray(images, dtype=np.float32)
+    N, H, W = images.shape
+    with Context(L, W, H, N, n_slots=N, precision=precision, kernel=kernel,
+                 views_per_pass=views_per_pass, run_length=run_length,
+                 clip=ranges is not None) as ctx:
+        if ranges is not None:
+            ctx.set_ranges(ranges)
+        if volume is not None:
+            ctx.set_volume(volume)
+        for n in range(N):
+            ctx.load_view(n, n + 1, matrices[n], images[n])
+        ctx.backproject(0, N)
+        return ctx.get_volume()
+
+
+# ---- runner loop mirror ---------------------------------------------------
+
+def run_module(rct_path, L, buffer_size=1, geometry=True, clip_file=None,
+               algorithm="LolaHIP", volume=None):
+    """Drive a module like rabbitRunner.c:187-287: zeroed L^3 volume, optional
+    global geometry (-a), prepare, buffered reads of b views per iterate call,
+    finish.  Returns the float32 (L,L,L) volume."""
+    prepare, iterate, finish = algorithm_registry_find(algorithm)
+    reader = rct.RctReader(rct_path)
+    h = reader.header
+    if h.n_images % buffer_size:
+        raise IOError("Failed to read projection matrix! Exit.")
+    vol = np.zeros((L, L, L), dtype=np.float32) if volume is None else volume
+    data = RabbitCtGlobalData()
+    data.voxelSize = voxel_size(L)
+    data.problemSize = L
+    data.O_Index = volume_origin(L)
+    data.volumeData = vol.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    data.imageWidth, data.imageHeight = h.width, h.height
+    data.numberOfProjections = 0
+    data.clipFilename = None if clip_file is None else os.fsencode(clip_file)
+    geom = None
+    if geometry:
+        geom = np.ascontiguousarray(reader.matrices(), dtype=np.float64)
+        data.numberOfProjections = h.n_images
+        data.globalGeometry = geom.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    prepare(ctypes.byref(data))
+    data.numberOfProjections = buffer_size
+    mats = [np.zeros(12, dtype=np.float64) for _ in range(buffer_size)]
+    imgs = [np.zeros((h.height, h.width), dtype=np.float32) for _ in range(buffer_size)]
+    buf = (Projection * buffer_size)()
+    for i in range(buffer_size):
+        buf[i].matrix = mats[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        buf[i].image = imgs[i].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    data.projectionBuffer = buf
+    counter = h.n_images
+    while counter > 0:
+        for i in range(buffer_size):
+            buf[i].id = reader.read_next(mats[i], imgs[i])
+        iterate(ctypes.byref(data))
+        counter -= buffer_size
+    finish(ctypes.byref(data))
+    return vol

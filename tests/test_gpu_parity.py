@@ -1,0 +1,157 @@
+"""GPU parity of the HIP path, through the C-ABI (ctypes), against the golden
+vectors of the compiled reference and the FP64 oracle.
+
+Tolerances: FP32 projections must agree with LolaOPT / LolaBunny to
+max |diff| <= 1e-4 of the volume range (the stated bar); half-precision
+storage must keep RMSE <= 1e-3 of the range against FP64."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import compare_summary, value_range
+from oracle import oracle as orc
+from rabbitct_amd import lolahip, rct
+from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry, voxel_size, volume_origin
+
+pytestmark = pytest.mark.gpu
+
+L = 128
+TOL = 1e-4
+
+
+def _bp(g, clip="clip16", **kw):
+    ranges = None if clip is None else g[clip]
+    return lolahip.backproject_views(L, g["matrices"], g["images"], ranges, **kw)
+
+
+@pytest.mark.parametrize("kernel,vpp,run", [("plain", 1, 16), ("fused", 12, 16), ("fused", 5, 8),
+                                            ("fused", 12, 32), ("fused", 1, 16)])
+def test_fp32_matches_lolaopt(golden, kernel, vpp, run):
+    v = _bp(golden, kernel=kernel, views_per_pass=vpp, run_length=run)
+    pt, zs = compare_summary(v, golden, "opt16")
+    print(f"\n{kernel} vpp={vpp} R={run}: max|d|/range vs LolaOPT {pt:.2e}, zsum {zs:.2e}")
+    assert pt <= TOL and zs <= TOL
+
+
+def test_fp32_matches_lolaopt_vs8(golden):
+    v = _bp(golden, clip="clip8")
+    pt, zs = compare_summary(v, golden, "opt8")
+    assert pt <= TOL and zs <= TOL
+
+
+def test_unclipped_matches_lolabunny(golden):
+    v = _bp(golden, clip=None)
+    pt, zs = compare_summary(v, golden, "bunny")
+    print(f"\nunclipped vs LolaBunny {pt:.2e}")
+    assert pt <= TOL and zs <= TOL
+
+
+def test_fp32_against_fp64_oracle_full_volume(golden):
+    v = _bp(golden)
+    ref = orc.backproject(L, voxel_size(L), volume_origin(L), golden["matrices"],
+                          golden["images"], golden["clip16"])
+    rng = float(ref.max() - ref.min())
+    err = np.abs(v - ref)
+    print(f"\nFP32 vs FP64: max {err.max() / rng:.2e}  rmse {np.sqrt((err ** 2).mean()) / rng:.2e}")
+    assert err.max() / rng <= TOL
+
+
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+def test_half_storage_error_vs_fp64(golden_border, prec):
+    g = golden_border
+    v = _bp(g, precision=prec)
+    ref = orc.backproject(L, voxel_size(L), volume_origin(L), g["matrices"], g["images"],
+                          g["clip16"])
+    rng = float(ref.max() - ref.min())
+    rmse = np.sqrt(((v - ref) ** 2).mean()) / rng
+    mx = np.abs(v - ref).max() / rng
+    print(f"\n{prec}: rmse {rmse:.2e} max {mx:.2e} of range vs FP64")
+    assert rmse <= 1e-3
+
+
+def test_accumulates_into_existing_volume(golden_border):
+    g = golden_border
+    rng = np.random.default_rng(0)
+    v0 = rng.normal(size=(L, L, L)).astype(np.float32)
+    a = _bp(g)
+    b = lolahip.backproject_views(L, g["matrices"], g["images"], g["clip16"], volume=v0)
+    np.testing.assert_allclose(b, v0 + a, atol=2e-6 * (np.abs(v0).max() + np.abs(a).max()))
+
+
+def test_zero_images_give_zero_volume(golden_border):
+    g = dict(golden_border)
+    g["images"] = np.zeros_like(g["images"])
+    assert not _bp(g).any()
+
+
+def test_device_line_ranges_byte_equal_reference(golden):
+    N, H, W = golden["images"].shape
+    for vs in (16, 8):
+        r = lolahip.compute_line_ranges(L, W, H, golden["matrices"], vs, device=True)
+        assert np.array_equal(r, golden[f"clip{vs}"])
+
+
+def test_device_line_ranges_full_detector_l512():
+    g = ScanGeometry(n_views=2, seed=21).build()
+    a = lolahip.compute_line_ranges(512, g.width, g.height, g.matrices, 16, device=True)
+    b = lolahip.compute_line_ranges(512, g.width, g.height, g.matrices, 16, device=False)
+    assert np.array_equal(a, b)
+
+
+def test_module_through_runner_loop(tmp_path, golden):
+    g = golden
+    N, H, W = g["images"].shape
+    p = tmp_path / "in.rct"
+    rct.write_rct(p, rct.RctHeader(W, H, N, (1000.0, 0.0)), g["matrices"], g["images"])
+    clip = tmp_path / "clip16.rct"
+    rct.write_clip_file(clip, g["clip16"])
+    for b in (1, 4, 12):
+        for vpp in ("1", "5", "32"):
+            os.environ["LOLAHIP_VIEWS_PER_PASS"] = vpp
+            try:
+                v = lolahip.run_module(p, L, buffer_size=b, clip_file=clip)
+            finally:
+                del os.environ["LOLAHIP_VIEWS_PER_PASS"]
+            pt, zs = compare_summary(v, g, "opt16")
+            assert pt <= TOL and zs <= TOL, (b, vpp, pt)
+
+
+def test_reference_runner_drives_lolahip(tmp_path, golden_border):
+    """The reference's own rabbitRunner (compiled from its sources, registry
+    with the LolaHIP row) drives the module unchanged: -m LolaHIP -a -C -b -o."""
+    runner = orc.ref_runner(orc.host_isa(), lolahip=True) or orc.ref_runner("avx2", lolahip=True)
+    if runner is None:
+        pytest.skip("reference runner with LolaHIP not built")
+    g = golden_border
+    N, H, W = g["images"].shape
+    rct.write_rct(tmp_path / "in.rct", rct.RctHeader(W, H, N, (1000.0, 0.0)), g["matrices"],
+                  g["images"])
+    rct.write_geometry(tmp_path / "geo.rct", rct.RctHeader(W, H, N, (1000.0, 0.0)), g["matrices"])
+    rct.write_clip_file(tmp_path / "clip16.rct", g["clip16"])
+    r = subprocess.run([runner, "-i", "in.rct", "-a", "geo.rct", "-m", "LolaHIP", "-s", "128",
+                        "-b", "4", "-C", "clip16.rct", "-o", "hip.vol"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    v = np.fromfile(tmp_path / "hip.vol", dtype=np.float32).reshape(L, L, L)
+    pt, zs = compare_summary(v, g, "opt16")
+    assert pt <= TOL and zs <= TOL
+
+
+def test_full_detector_l256_sampled_lines_vs_oracle():
+    """Full 1248x960 detector at L=256: GPU volume vs the FP64 oracle on the
+    same views (the oracle run restricted to a few views)."""
+    geom = ScanGeometry(n_views=3, seed=17).build()
+    ph = EllipsoidPhantom(seed=17).build()
+    rng = np.random.default_rng(2)
+    imgs = rng.normal(size=(3, geom.height, geom.width)).astype(np.float32)
+    imgs[:, 0, :] = imgs[:, -1, :] = imgs[:, :, 0] = imgs[:, :, -1] = 0
+    L2 = 256
+    ranges = lolahip.compute_line_ranges(L2, geom.width, geom.height, geom.matrices, 16)
+    v = lolahip.backproject_views(L2, geom.matrices, imgs, ranges)
+    ref = orc.backproject(L2, voxel_size(L2), volume_origin(L2), geom.matrices, imgs, ranges)
+    rngv = float(ref.max() - ref.min())
+    err = np.abs(v - ref).max() / rngv
+    print(f"\nL=256 full detector: max|d|/range {err:.2e}")
+    assert err <= TOL

@@ -1,0 +1,245 @@
+"""Pins the CPU oracle (oracle/rct_oracle.c) before anything is compared with
+it: against what the compiled reference produced (tests/golden, written by
+tests/golden/make_golden.py), and against closed forms / brute force where
+the reference holds no answer."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import compare_summary, value_range
+from oracle import oracle as orc
+from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry, voxel_size, volume_origin
+
+L = 128
+RL, OL = voxel_size(L), volume_origin(L)
+
+
+def _dims(g):
+    N, H, W = g["images"].shape
+    return N, W, H
+
+
+# ---- against the compiled reference ------------------------------------
+
+@pytest.mark.parametrize("vs", [16, 8])
+def test_line_ranges_byte_equal_reference(golden, vs):
+    N, W, H = _dims(golden)
+    r = orc.line_ranges(L, RL, OL, W, H, golden["matrices"], vs)
+    assert np.array_equal(r, golden[f"clip{vs}"])
+
+
+def test_shadow_matches_asm_padding(golden):
+    N, W, H = _dims(golden)
+    s = orc.shadow(L, RL, OL, W, H, golden["matrices"])
+    up16 = lambda v: (v + 15) // 16 * 16                          # LolaASM.c:53-57
+    pads = [up16(abs(s["Vmin"])), up16(abs(s["Vmax"]) - H), up16(abs(s["Umin"])),
+            up16(abs(s["Umax"]) - W)]
+    assert pads == list(golden["shadow"])
+
+
+def test_bunny_matches_reference(golden):
+    v = orc.backproject(L, RL, OL, golden["matrices"], golden["images"], mode="bunny")
+    pt, zs = compare_summary(v.astype(np.float32), golden, "bunny")
+    assert pt < 1e-6 and zs < 1e-6
+
+
+@pytest.mark.parametrize("alg,clip", [("opt16", "clip16"), ("omp16", "clip16"), ("opt8", "clip8")])
+def test_clipped_matches_reference(golden, alg, clip):
+    v = orc.backproject(L, RL, OL, golden["matrices"], golden["images"], golden[clip])
+    pt, zs = compare_summary(v.astype(np.float32), golden, alg)
+    assert pt < 2e-5 and zs < 2e-6
+
+
+def test_asm_semantics_match_reference(golden):
+    v = orc.backproject(L, RL, OL, golden["matrices"], golden["images"], golden["clip16"],
+                        mode="asm", vs=16)
+    pt, zs = compare_summary(v.astype(np.float32), golden, "asm16")
+    assert pt < 2e-5 and zs < 2e-6
+
+
+def test_noborder_fixture_separates_bunny_from_clipped(golden_noborder):
+    """Without a zero detector border LolaBunny and LolaOPT differ in the band
+    trunc(u) = -1: the fixture distinguishes the two semantics."""
+    g = golden_noborder
+    d = np.abs(g["bunny_slice"] - g["opt16_slice"]).max() / value_range(g)
+    assert d > 1e-3
+
+
+def test_border_fixture_unifies_semantics(golden_border):
+    g = golden_border
+    d = np.abs(g["bunny_samples"] - g["opt16_samples"]).max() / value_range(g)
+    assert d < 1e-4
+
+
+def test_hu_conversion_matches_runner_pgm(golden):
+    h0, h1 = golden["hu"]
+    hu = orc.hu_convert(golden["opt16_slice"], h0, h1)
+    assert np.array_equal(hu, golden["pgm_slice"])
+
+
+def test_hu_metric_matches_runner(golden):
+    """The runner's -c numbers for a reference volume = own HU + pert: any
+    volume gives the same metric against HU(volume) + pert."""
+    rng = np.random.default_rng(0)
+    vol = rng.uniform(-0.5, 3.0, L ** 3).astype(np.float32)
+    h0, h1 = golden["hu"]
+    ref = orc.hu_convert(vol, h0, h1)
+    ref[golden["pert_idx"]] += golden["pert_val"]
+    m = orc.hu_metric(vol, ref, h0, h1)
+    assert m["mse"] == pytest.approx(float(golden["check_mse"]), rel=1e-5)
+    assert m["max"] == float(golden["check_max"])
+    assert m["psnr"] == pytest.approx(float(golden["check_psnr"]), rel=1e-5)
+    assert np.array_equal(m["hist"], golden["check_hist"])
+
+
+# ---- closed forms and invariants ----------------------------------------
+
+@pytest.fixture(scope="module")
+def small_geom():
+    return ScanGeometry(n_views=4, width=104, height=80, pixel=3.6, seed=3).build()
+
+
+def _project(a, x, y, z):
+    P = a.reshape(4, 3).T
+    w = P[2, 0] * x + P[2, 1] * y + P[2, 2] * z + P[2, 3]
+    u = (P[0, 0] * x + P[0, 1] * y + P[0, 2] * z + P[0, 3]) / w
+    v = (P[1, 0] * x + P[1, 1] * y + P[1, 2] * z + P[1, 3]) / w
+    return u, v, w
+
+
+def _grid():
+    c = OL + np.arange(L) * RL
+    return np.meshgrid(c, c, c, indexing="ij")   # z, y, x
+
+
+def test_constant_image_gives_sum_of_inverse_square_depth(small_geom):
+    g = small_geom
+    W, H = g.width, g.height
+    img = np.zeros((g.n_views, H, W), np.float32)
+    img[:, 1:H - 1, 1:W - 1] = 1.5
+    vol = orc.backproject(L, RL, OL, g.matrices, img, mode="bunny")
+    z, y, x = _grid()
+    expect = np.zeros_like(vol)
+    inside = np.ones(vol.shape, bool)
+    for n in range(g.n_views):
+        u, v, w = _project(g.matrices[n], x, y, z)
+        expect += 1.5 / w ** 2
+        inside &= (u >= 1) & (u < W - 3) & (v >= 1) & (v < H - 3)
+    assert inside.sum() > 1000
+    np.testing.assert_allclose(vol[inside], expect[inside], rtol=1e-12)
+
+
+def test_single_pixel_gives_tent_weight(small_geom):
+    g = small_geom
+    a = g.matrices[1]
+    i0, j0 = 50, 37
+    img = np.zeros((1, g.height, g.width), np.float32)
+    img[0, j0, i0] = 1.0
+    vol = orc.backproject(L, RL, OL, a[None], img, mode="bunny")
+    z, y, x = _grid()
+    u, v, w = _project(a, x, y, z)
+    tent = np.clip(1 - np.abs(u - i0), 0, None) * np.clip(1 - np.abs(v - j0), 0, None)
+    np.testing.assert_allclose(vol, tent / w ** 2, atol=1e-15)
+    assert (vol > 0).sum() > 10
+
+
+def test_truncation_extrapolates_left_of_column_zero(small_geom):
+    """u in (-1,0): trunc gives column 0 with weight 1-u > 1 and column 1 with
+    weight u < 0 (LolaBunny.c:19-30)."""
+    g = small_geom
+    a = g.matrices[0]
+    z, y, x = _grid()
+    u, v, w = _project(a, x, y, z)
+    sel = (u > -1) & (u < 0) & (v > 5) & (v < g.height - 6)
+    assert sel.sum() > 0
+    img = np.zeros((1, g.height, g.width), np.float32)
+    img[0, :, 0] = 1.0
+    img[0, :, 1] = 3.0
+    vol = orc.backproject(L, RL, OL, a[None], img, mode="bunny")
+    expect = ((1 - u) * 1.0 + u * 3.0) / w ** 2
+    np.testing.assert_allclose(vol[sel], expect[sel], rtol=1e-12)
+
+
+def test_matrix_scaling_scales_volume(small_geom, golden_border):
+    g = golden_border
+    s = 4.0
+    v1 = orc.backproject(L, RL, OL, g["matrices"][:3], g["images"][:3], g["clip16"][:3])
+    v2 = orc.backproject(L, RL, OL, g["matrices"][:3] * s, g["images"][:3], g["clip16"][:3])
+    np.testing.assert_allclose(v2 * s * s, v1, rtol=1e-12, atol=1e-15)
+
+
+def test_linearity_and_view_order(golden_border):
+    g = golden_border
+    m, im, r = g["matrices"][:4], g["images"][:4], g["clip16"][:4]
+    rng = np.random.default_rng(1)
+    im2 = rng.normal(size=im.shape).astype(np.float32)
+    a = orc.backproject(L, RL, OL, m, im, r)
+    b = orc.backproject(L, RL, OL, m, im2, r)
+    c = orc.backproject(L, RL, OL, m, (2 * im + 3 * im2).astype(np.float32), r)
+    # the combined image is rounded to float32: 2^-24 relative per pixel
+    np.testing.assert_allclose(c, 2 * a + 3 * b, atol=1e-5 * np.abs(c).max())
+    perm = [2, 0, 3, 1]
+    d = orc.backproject(L, RL, OL, m[perm], im[perm], r[perm])
+    np.testing.assert_allclose(d, a, atol=1e-12 * np.abs(a).max())
+
+
+def test_clipped_equals_bunny_inside_ranges_with_border(golden_border):
+    g = golden_border
+    vb = orc.backproject(L, RL, OL, g["matrices"][:2], g["images"][:2], mode="bunny")
+    vc = orc.backproject(L, RL, OL, g["matrices"][:2], g["images"][:2], g["clip16"][:2])
+    np.testing.assert_allclose(vc, vb, atol=1e-12 * np.abs(vb).max())
+
+
+# ---- the generator's FP64 form ------------------------------------------
+
+def test_sphere_chord_closed_form():
+    g = ScanGeometry(n_views=3, width=40, height=30, pixel=9.0, seed=2).build()
+    S, Minv, _ = g.rays()
+    r, c = 50.0, np.array([10.0, -5.0, 3.0])
+    ell = np.zeros((1, 16))
+    ell[0, 0:3] = c
+    ell[0, 3:6] = r
+    ell[0, 6] = 1.0
+    ell[0, 7:16] = np.eye(3).ravel()
+    p = orc.line_integrals(S, Minv, ell, g.width, g.height)
+    ii, jj = np.meshgrid(np.arange(g.width, dtype=float), np.arange(g.height, dtype=float))
+    for n in range(3):
+        d = np.stack([ii, jj, np.ones_like(ii)], -1) @ Minv[n].T
+        d /= np.linalg.norm(d, axis=-1, keepdims=True)
+        t = (c - S[n]) @ d.reshape(-1, 3).T
+        dist2 = ((c - S[n]) ** 2).sum() - t ** 2
+        expect = 2 * np.sqrt(np.clip(r * r - dist2, 0, None)).reshape(g.height, g.width)
+        np.testing.assert_allclose(p[n], expect, atol=1e-9)
+    assert (p > 0).sum() > 100
+
+
+def test_ellipsoid_integrals_match_ray_marching():
+    g = ScanGeometry(n_views=2, width=12, height=10, pixel=30.0, seed=4).build()
+    ph = EllipsoidPhantom(n_ell=3, seed=4).build()
+    S, Minv, _ = g.rays()
+    p = orc.line_integrals(S, Minv, ph.params, g.width, g.height)
+    step = 0.05
+    t = np.arange(0.0, 1500.0, step) + step / 2
+    for n in range(2):
+        for j in range(0, g.height, 3):
+            for i in range(0, g.width, 3):
+                d = Minv[n] @ np.array([i, j, 1.0])
+                d /= np.linalg.norm(d)
+                pts = S[n][None] + t[:, None] * d[None]
+                dens = ph.density(pts[:, 0], pts[:, 1], pts[:, 2])
+                assert p[n, j, i] == pytest.approx(dens.sum() * step, abs=0.6)
+
+
+def test_ramp_filter_matches_fft_convolution():
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(5, 64))
+    tau = 0.3
+    y = orc.ramp_filter_rows(x, tau)
+    k = np.arange(-63, 64)
+    h = np.where(k == 0, 1 / (4 * tau * tau),
+                 np.where(k % 2 == 1, -1 / (math.pi ** 2 * k.astype(float) ** 2 * tau * tau), 0.0))
+    h[np.abs(k) % 2 == 1] = -1 / (math.pi ** 2 * k[np.abs(k) % 2 == 1].astype(float) ** 2 * tau ** 2)
+    n = 256
+    Y = np.fft.irfft(np.fft.rfft(x, n) * np.fft.rfft(np.roll(np.pad(h, (0, n - h.size)), -63), n), n)
+    np.testing.assert_allclose(y, tau * Y[:, :64], atol=1e-9)
