@@ -139,6 +139,7 @@ typedef struct {
   uint32_t viewsPerPass; /* fused: views accumulated per register pass */
   uint32_t runLength;  /* voxels along x per thread: 8, 16 or 32 (0 = auto) */
   int clip;            /* 1: line ranges (LolaOPT semantics), 0: all voxels */
+  int staging;         /* 1: per-view footprints staged in LDS (FP32), 0: L2 gathers */
   int device;          /* HIP device ordinal */
 } LhConfig;
 
@@ -181,6 +182,16 @@ int lhTimedBackproject(LhContext *ctx, uint32_t count, int reps, float *msPerRep
     float *msPerLaunch, uint32_t *launchesPerRep);
 
 int lhSynchronize(LhContext *ctx);
+
+/* Changes the launch shape of later backprojections (kernel, views per
+ * register pass, voxels per thread along x, LDS staging) without touching
+ * resident data; the autotuner's knob. */
+int lhSetLaunch(LhContext *ctx, int kernel, uint32_t viewsPerPass, uint32_t runLength,
+    int staging);
+
+/* Device-to-device copy of `bytes` on the context's stream, timed with HIP
+ * events: the reference copy against which HBM counters are scaled. */
+int lhCopyCalibration(LhContext *ctx, uint64_t bytes, float *ms);
 
 /* Effective voxel-view updates of views [0,count) of the slots: the number of
  * voxels inside their line ranges (L^3 per view without clipping). */

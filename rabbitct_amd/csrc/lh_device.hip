@@ -247,15 +247,270 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
   }
 }
 
+// ---- LDS-staged tile ------------------------------------------------------
+// Tile of R (x) x 32 (y) x 8 (z) voxels: lane l of wave w owns
+// (x0..x0+R-1, y0 + l%32, z0 + 2w + l/32).  Per view the workgroup projects
+// the tile's 8 corners, takes the bounding box of their detector positions
+// (plus a 1-pixel margin and the right/lower tap) and copies that footprint of
+// the bordered projection into LDS with direct-to-LDS buffer loads (one
+// instruction moves 64 consecutive pixels of a row, no VGPR round trip).  The
+// footprints are double-buffered: view v+1 streams in while view v computes,
+// one barrier per view.  Voxel coordinates are clamped to the box, so every
+// LDS index is in bounds by construction (for voxels inside the tile the
+// clamp is the identity).  A footprint larger than kFW x kFH falls back to
+// L2 gathers for that view.
+constexpr int kFW    = 128; // LDS row pitch (floats); ds_read2 offsets stay <= 255
+constexpr int kFH    = 39;  // LDS rows per buffer (2 buffers x 39 x 128 x 4 B: 4 workgroups per CU)
+constexpr int kLYs   = 32;  // lanes along y in the staged tile
+constexpr int kTileZ = 8;   // z extent of the staged tile
+
+struct StageBox {
+  int iu0, iv0;   // footprint origin (pixels)
+  int fw, fh;     // footprint size
+  float lou, hiu, lov, hiv; // clamp bounds of the voxel coordinates
+  bool fits;
+};
+
+// Footprint of the tile's corners for the view with FP64 matrix a; lanes
+// (l & 7) project corner l & 7, the min/max is reduced across each group of 8
+// lanes, so every lane (and every wave) ends with the same box.
+__device__ __forceinline__ StageBox stage_box(const double *a, double cx0, double cx1,
+    double cy0, double cy1, double cz0, double cz1, float Wf, float Hf)
+{
+  const int c    = threadIdx.x & 7;
+  const double x = (c & 1) ? cx1 : cx0;
+  const double y = (c & 2) ? cy1 : cy0;
+  const double z = (c & 4) ? cz1 : cz0;
+  const float hu = (float)fma(a[0], x, fma(a[3], y, fma(a[6], z, a[9])));
+  const float hv = (float)fma(a[1], x, fma(a[4], y, fma(a[7], z, a[10])));
+  const float hw = (float)fma(a[2], x, fma(a[5], y, fma(a[8], z, a[11])));
+  const float rw = __builtin_amdgcn_rcpf(hw);
+  float umin = __builtin_amdgcn_fmed3f(hu * rw, -2.0f, Wf), umax = umin;
+  float vmin = __builtin_amdgcn_fmed3f(hv * rw, -2.0f, Hf), vmax = vmin;
+  for (int o = 1; o < 8; o <<= 1) {
+    umin = fminf(umin, __shfl_xor(umin, o));
+    umax = fmaxf(umax, __shfl_xor(umax, o));
+    vmin = fminf(vmin, __shfl_xor(vmin, o));
+    vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+  }
+  StageBox b;
+  const int W = (int)Wf, H = (int)Hf;
+  // taps of a coordinate t lie in [trunc(t), trunc(t)+1] within
+  // [floor(t), floor(t)+2]; one pixel of margin on each side for rounding
+  const int iu0 = max((int)floorf(umin) - 1, -2);
+  const int iv0 = max((int)floorf(vmin) - 1, -2);
+  const int iu1 = min((int)floorf(umax) + 3, W + 2);
+  const int iv1 = min((int)floorf(vmax) + 3, H + 2);
+  b.iu0  = __builtin_amdgcn_readfirstlane(iu0);
+  b.iv0  = __builtin_amdgcn_readfirstlane(iv0);
+  b.fw   = __builtin_amdgcn_readfirstlane(iu1 - iu0);
+  b.fh   = __builtin_amdgcn_readfirstlane(iv1 - iv0);
+  b.fits = b.fw <= kFW && b.fh <= kFH && b.fw >= 2 && b.fh >= 2;
+  if (b.fits) {
+    b.lou = (float)b.iu0;
+    b.hiu = (float)(b.iu0 + b.fw - 2);
+    b.lov = (float)b.iv0;
+    b.hiv = (float)(b.iv0 + b.fh - 2);
+  } else {
+    b.lou = -2.0f;
+    b.hiu = Wf;
+    b.lov = -2.0f;
+    b.hiv = Hf;
+  }
+  return b;
+}
+
+// Issues the direct-to-LDS loads of footprint `b` of the slot image into buf:
+// wave w copies rows w, w+4, ...; each instruction moves pixels
+// [iu0 + c0, iu0 + c0 + 64) of one row into buf[r*kFW + c0 ..].  Columns at or
+// beyond fw are copied too and never read; past the end of the slot the
+// buffer bounds return zero.  Completion is awaited by the caller
+// (s_waitcnt vmcnt(0) and a barrier).
+__device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StageBox &b,
+    int pitch, float *buf)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv   = threadIdx.x / kWave;
+  const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * 4u;
+  for (int r = wv; r < b.fh; r += kWaves) {
+    const uint32_t row = base + (uint32_t)(r * pitch) * 4u;
+    for (int c0 = 0; c0 < b.fw; c0 += kWave) {
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r * kFW + c0), 4,
+          (int)(row + (uint32_t)(c0 + lane) * 4u), 0, 0, 0);
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
+    int *sAct)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv   = threadIdx.x / kWave;
+  const int x0   = tx * R;
+  const int y0   = ty * kLYs;
+  const int z0   = tz * kTileZ;
+  const int y    = y0 + (lane & (kLYs - 1));
+  const int z    = z0 + 2 * wv + lane / kLYs;
+  const int L    = A.L;
+  const double O = A.O_L, RL = A.R_L;
+
+  const double wx0 = O + (double)x0 * RL;
+  const double wy  = O + (double)y * RL;
+  const double wz  = O + (double)z * RL;
+  const double cx1 = O + (double)(x0 + R - 1) * RL;
+  const double cy0 = O + (double)y0 * RL, cy1 = O + (double)(y0 + kLYs - 1) * RL;
+  const double cz0 = O + (double)z0 * RL, cz1 = O + (double)(z0 + kTileZ - 1) * RL;
+  const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
+  const uint64_t LL   = (uint64_t)L * (uint64_t)L;
+  const int pitch     = (int)(A.rowBytes >> 2);
+  const int n         = A.nViews;
+
+  auto range_of = [&](int v) -> int {
+    return A.clip ? A.ranges[(uint64_t)(A.views[A.firstSlot + v].id - 1) * LL + line] : (L << 16);
+  };
+  auto lane_active = [&](int rr) -> bool {
+    const int start = (int)(short)(rr & 0xffff), stop = rr >> 16;
+    return max(start - x0, 0) < min(stop - x0, R);
+  };
+
+  float acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    acc[r] = 0.0f;
+  bool touched = false;
+
+  // prologue: view 0 activity and footprint
+  int rr_cur = range_of(0);
+  int rr_nxt = n > 1 ? range_of(1) : 0;
+  // the vote runs on the whole wave, only the store is lane 0's
+  const bool act0 = __any(lane_active(rr_cur));
+  if (lane == 0)
+    sAct[wv] = act0 ? 1 : 0;
+  __syncthreads();
+  bool blockAct = (sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
+  StageBox box  = stage_box(A.views[A.firstSlot].a, wx0, cx1, cy0, cy1, cz0, cz1, A.Wf, A.Hf);
+  if (blockAct && box.fits)
+    stage_issue(slot_rsrc(A, A.firstSlot), box, pitch, sImg);
+
+  for (int v = 0; v < n; ++v) {
+    const int slot      = A.firstSlot + v;
+    const ViewParams &P = A.views[slot];
+    const bool hasNext  = v + 1 < n;
+    // footprint(v) and the range loads have landed; publish activity of v+1
+    __builtin_amdgcn_s_waitcnt(0);
+    const bool actNext = hasNext && __any(lane_active(rr_nxt));
+    if (lane == 0)
+      sAct[((v + 1) & 1) * kWaves + wv] = actNext ? 1 : 0;
+    __syncthreads();
+    const int *an = sAct + ((v + 1) & 1) * kWaves;
+    const bool blockActNext = (an[0] | an[1] | an[2] | an[3]) != 0;
+    const int rr   = rr_cur;
+    rr_cur         = rr_nxt;
+    if (v + 2 < n)
+      rr_nxt = range_of(v + 2);
+    StageBox boxNext = box;
+    if (hasNext) {
+      boxNext = stage_box(A.views[slot + 1].a, wx0, cx1, cy0, cy1, cz0, cz1, A.Wf, A.Hf);
+      if (blockActNext && boxNext.fits)
+        stage_issue(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kFW * kFH);
+    }
+
+    const int start   = (int)(short)(rr & 0xffff);
+    const int stop    = rr >> 16;
+    const int rlo     = max(start - x0, 0);
+    const int rhi     = min(stop - x0, R);
+    const bool active = rlo < rhi;
+    if (blockAct && __any(active)) {
+      touched |= active;
+      const float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
+      const float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
+      const float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
+      const float du = P.du, dv = P.dv, dw = P.dw;
+      if (box.fits) {
+        // sBase[v*kFW + u] is footprint pixel (u, v): the window origin folded in
+        const float *sBase = sImg + (v & 1) * kFW * kFH - (box.iv0 * kFW + box.iu0);
+        auto update = [&](int r) -> float {
+          const float fr = (float)r;
+          const float rw = __builtin_amdgcn_rcpf(fmaf(fr, dw, w0));
+          const float ix = __builtin_amdgcn_fmed3f(fmaf(fr, du, u0) * rw, box.lou, box.hiu);
+          const float iy = __builtin_amdgcn_fmed3f(fmaf(fr, dv, v0) * rw, box.lov, box.hiv);
+          const float fx = __builtin_truncf(ix);
+          const float fy = __builtin_truncf(iy);
+          // fy*kFW + fx is an exact integer in FP32 (|.| < 2^24): one conversion
+          const float *q  = sBase + (int)fmaf(fy, (float)kFW, fx);
+          const float p00 = q[0], p10 = q[1];
+          const float p01 = q[kFW], p11 = q[kFW + 1];
+          const float ax = ix - fx, ay = iy - fy;
+          const float b   = fmaf(ax, p10 - p00, p00);
+          const float t   = fmaf(ax, p11 - p01, p01);
+          return fmaf(ay, t - b, b) * (rw * rw);
+        };
+        // groups of four updates are fenced so the scheduler keeps at most four
+        // updates' taps in flight per lane
+        if (__all(rlo == 0 && rhi == R)) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            acc[r] += update(r);
+            if ((r & 3) == 3)
+              __builtin_amdgcn_sched_barrier(0);
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            if (r >= rlo && r < rhi)
+              acc[r] += update(r);
+            if ((r & 3) == 3)
+              __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      } else {
+        // footprint larger than the LDS window: L2 gathers, one voxel at a time
+        const __amdgpu_buffer_rsrc_t rs = slot_rsrc(A, slot);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r >= rlo && r < rhi) {
+            const float fr = (float)r;
+            acc[r] += voxel_update<LH_PREC_F32>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0),
+                fmaf(fr, dw, w0));
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    box      = boxNext;
+    blockAct = blockActNext;
+  }
+
+  if (touched) {
+    float4 *p = reinterpret_cast<float4 *>(A.vol + line * (uint64_t)L + (uint64_t)x0);
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) {
+      float4 o = p[q];
+      o.x += acc[4 * q + 0];
+      o.y += acc[4 * q + 1];
+      o.z += acc[4 * q + 2];
+      o.w += acc[4 * q + 3];
+      p[q] = o;
+    }
+  }
+}
+
 // Grid form: one workgroup per tile.  Persistent form: a grid sized to the
 // chip walks the tiles; tile t goes to XCD (t mod 8) in dispatch order, so the
 // tiles are renumbered such that each XCD's share is one contiguous z-slab and
 // its L2 serves one band of detector rows.
-template <int R, int PREC>
-__global__ __launch_bounds__(kWaves *kWave) void bp_kernel(BpArgs A)
+template <int R, int PREC, int STAGE>
+__global__ __launch_bounds__(kWaves *kWave, 4) void bp_kernel(BpArgs A)
 {
+  __shared__ float sImg[STAGE ? 2 * kFW * kFH : 1];
+  __shared__ int sAct[2 * kWaves];
   if (!A.persistent) {
-    bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+    if constexpr (STAGE)
+      bp_tile_lds<R>(A, blockIdx.x, blockIdx.y, blockIdx.z, sImg, sAct);
+    else
+      bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
     return;
   }
   const int nTiles  = A.tilesX * A.tilesY * A.tilesZ;
@@ -270,7 +525,12 @@ __global__ __launch_bounds__(kWaves *kWave) void bp_kernel(BpArgs A)
     const int tx = t % A.tilesX;
     const int ty = (t / A.tilesX) % A.tilesY;
     const int tz = t / (A.tilesX * A.tilesY);
-    bp_tile<R, PREC>(A, tx, ty, tz);
+    if constexpr (STAGE) {
+      __syncthreads(); // the previous tile's last LDS reads are done
+      bp_tile_lds<R>(A, tx, ty, tz, sImg, sAct);
+    } else {
+      bp_tile<R, PREC>(A, tx, ty, tz);
+    }
   }
 }
 
@@ -439,6 +699,24 @@ struct LhContext {
   std::vector<int> slotId;          // id held by each slot (0 = empty)
 };
 
+// Grid of one launch: one workgroup per tile, or for the persistent form as
+// many workgroups as the chip keeps resident (occupancy x CUs).
+template <int R, int PREC, int STAGE>
+static void launch_shape(LhContext *c, const BpArgs &A)
+{
+  static int perCU = 0;
+  dim3 grid(A.tilesX, A.tilesY, A.tilesZ);
+  if (A.persistent) {
+    if (perCU == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU,
+                           bp_kernel<R, PREC, STAGE>, kWaves * kWave, 0) != hipSuccess ||
+                          perCU < 1))
+      perCU = 4;
+    grid = dim3((unsigned)std::min<long>((long)c->numCU * perCU,
+        (long)A.tilesX * A.tilesY * A.tilesZ));
+  }
+  bp_kernel<R, PREC, STAGE><<<grid, dim3(kWaves * kWave), 0, c->stream>>>(A);
+}
+
 const LhConfig &lh_config(const LhContext *c) { return c->cfg; }
 hipStream_t lh_stream(const LhContext *c) { return c->stream; }
 
@@ -476,6 +754,7 @@ void lhDefaultConfig(LhConfig *cfg, uint32_t L, uint32_t W, uint32_t H, uint32_t
   cfg->kernel       = LH_KERNEL_FUSED;
   cfg->runLength    = 16;
   cfg->clip         = 1;
+  cfg->staging      = 1;
   cfg->device       = 0;
 }
 
@@ -756,23 +1035,19 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
   A.nViews      = (int)count;
   A.clip        = g.clip;
   const int R   = (int)g.runLength;
+  const int stage = (g.staging && g.precision == LH_PREC_F32) ? 1 : 0;
   A.tilesX      = (int)g.L / R;
-  A.tilesY      = (int)g.L / kWave;
-  A.tilesZ      = (int)g.L / kWaves;
+  A.tilesY      = (int)g.L / (stage ? kLYs : kWave);
+  A.tilesZ      = (int)g.L / (stage ? kTileZ : kWaves);
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
-  dim3 grid;
-  if (A.persistent)
-    grid = dim3(std::min<long>((long)c->numCU * 8, (long)A.tilesX * A.tilesY * A.tilesZ));
-  else
-    grid = dim3(A.tilesX, A.tilesY, A.tilesZ);
-  dim3 block(kWaves * kWave);
-
-#define LH_LAUNCH(RR, PP) bp_kernel<RR, PP><<<grid, block, 0, c->stream>>>(A)
+#define LH_LAUNCH(RR, PP, SS) launch_shape<RR, PP, SS>(c, A)
 #define LH_BY_PREC(RR)                                                                   \
   switch (g.precision) {                                                                 \
-  case LH_PREC_F32: LH_LAUNCH(RR, LH_PREC_F32); break;                                    \
-  case LH_PREC_BF16: LH_LAUNCH(RR, LH_PREC_BF16); break;                                  \
-  default: LH_LAUNCH(RR, LH_PREC_F16); break;                                             \
+  case LH_PREC_F32:                                                                      \
+    if (stage) LH_LAUNCH(RR, LH_PREC_F32, 1); else LH_LAUNCH(RR, LH_PREC_F32, 0);          \
+    break;                                                                               \
+  case LH_PREC_BF16: LH_LAUNCH(RR, LH_PREC_BF16, 0); break;                               \
+  default: LH_LAUNCH(RR, LH_PREC_F16, 0); break;                                          \
   }
   switch (R) {
   case 8: LH_BY_PREC(8); break;
@@ -850,6 +1125,46 @@ int lhTimedBackproject(LhContext *c, uint32_t count, int reps, float *msPerRep,
     *msPerLaunch = (float)(launchSum / ((double)reps * nl));
   if (launchesPerRep)
     *launchesPerRep = nl;
+  return LH_OK;
+}
+
+int lhSetLaunch(LhContext *c, int kernel, uint32_t viewsPerPass, uint32_t runLength,
+    int staging)
+{
+  if (!c || (kernel != LH_KERNEL_PLAIN && kernel != LH_KERNEL_FUSED) || viewsPerPass == 0 ||
+      (runLength != 8 && runLength != 16 && runLength != 32) || c->cfg.L % runLength)
+    return fail(LH_E_ARG, "lhSetLaunch: bad launch shape");
+  c->cfg.kernel       = kernel;
+  c->cfg.viewsPerPass = viewsPerPass;
+  c->cfg.runLength    = runLength;
+  c->cfg.staging      = staging ? 1 : 0;
+  return LH_OK;
+}
+
+int lhCopyCalibration(LhContext *c, uint64_t bytes, float *ms)
+{
+  if (!c || bytes == 0)
+    return fail(LH_E_ARG, "lhCopyCalibration: bad arguments");
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  LH_HIP(hipMalloc(&a, bytes));
+  LH_HIP(hipMalloc(&b, bytes));
+  LH_HIP(hipMemsetAsync(a, 1, bytes, c->stream));
+  LH_HIP(hipEventCreate(&e0));
+  LH_HIP(hipEventCreate(&e1));
+  LH_HIP(hipEventRecord(e0, c->stream));
+  LH_HIP(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream));
+  LH_HIP(hipEventRecord(e1, c->stream));
+  LH_HIP(hipEventSynchronize(e1));
+  float t = 0.0f;
+  LH_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms)
+    *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return LH_OK;
 }
 

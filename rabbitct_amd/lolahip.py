@@ -62,7 +62,7 @@ class LhConfig(ctypes.Structure):
                 ("nSlots", ctypes.c_uint32), ("precision", ctypes.c_int),
                 ("kernel", ctypes.c_int), ("viewsPerPass", ctypes.c_uint32),
                 ("runLength", ctypes.c_uint32), ("clip", ctypes.c_int),
-                ("device", ctypes.c_int)]
+                ("staging", ctypes.c_int), ("device", ctypes.c_int)]
 
 
 # Every symbol include/lolahip.h declares.
@@ -73,6 +73,7 @@ EXPORTS = (
     "lhSetRangesFromGeometry", "lhSetVolume", "lhZeroVolume", "lhGetVolume",
     "lhLoadView", "lhLoadViewDevice", "lhBackproject", "lhTimedBackproject",
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
+    "lhSetLaunch", "lhCopyCalibration",
 )
 
 _lib = None
@@ -120,6 +121,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhLastError.restype = ctypes.c_char_p
     lib.lhGenerateViews.argtypes = [P, u32, u32, i32, P, P, P, P, P, u32, P, ctypes.c_double,
                                     ctypes.c_double, i32, P]
+    lib.lhSetLaunch.argtypes = [P, i32, u32, u32, i32]
+    lib.lhCopyCalibration.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(f32)]
     lib.lhAbiLayout.argtypes = [P]
     lib.lhAbiLayout.restype = None
     _lib = lib
@@ -187,7 +190,8 @@ class Context:
     resident in HBM."""
 
     def __init__(self, L, W, H, n_views, n_slots=None, precision="f32", kernel="fused",
-                 views_per_pass=32, run_length=16, clip=True, device=0, rL=None, oL=None):
+                 views_per_pass=32, run_length=16, clip=True, device=0, rL=None, oL=None,
+                 staging=True):
         lib = load_library()
         cfg = LhConfig()
         lib.lhDefaultConfig(ctypes.byref(cfg), L, W, H, n_views)
@@ -199,6 +203,7 @@ class Context:
         cfg.viewsPerPass = views_per_pass
         cfg.runLength = run_length
         cfg.clip = 1 if clip else 0
+        cfg.staging = 1 if staging else 0
         cfg.device = device
         self.cfg = cfg
         self._h = ctypes.c_void_p()
@@ -262,19 +267,32 @@ class Context:
                "lhTimedBackproject")
         return ms.value, msl.value, nl.value
 
+    def set_launch(self, kernel, views_per_pass, run_length, staging=True):
+        _check(self.lib.lhSetLaunch(self._h, KERNEL[kernel], views_per_pass, run_length,
+                                    1 if staging else 0), "lhSetLaunch")
+        self.cfg.kernel = KERNEL[kernel]
+        self.cfg.viewsPerPass = views_per_pass
+        self.cfg.runLength = run_length
+        self.cfg.staging = 1 if staging else 0
+
+    def copy_calibration(self, nbytes):
+        ms = ctypes.c_float()
+        _check(self.lib.lhCopyCalibration(self._h, nbytes, ctypes.byref(ms)), "lhCopyCalibration")
+        return ms.value
+
     def effective_updates(self, count):
         return int(self.lib.lhEffectiveUpdates(self._h, count))
 
 
 def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel="fused",
-                      views_per_pass=32, run_length=16, volume=None):
+                      views_per_pass=32, run_length=16, volume=None, staging=True):
     """Backproject views (N,12)/(N,H,W) on the GPU; returns the (L,L,L) float32
     volume (+= into ``volume`` when given).  ``ranges`` None = unclipped."""
     images = np.asarray(images, dtype=np.float32)
     N, H, W = images.shape
     with Context(L, W, H, N, n_slots=N, precision=precision, kernel=kernel,
                  views_per_pass=views_per_pass, run_length=run_length,
-                 clip=ranges is not None) as ctx:
+                 clip=ranges is not None, staging=staging) as ctx:
         if ranges is not None:
             ctx.set_ranges(ranges)
         if volume is not None:
