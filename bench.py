@@ -33,7 +33,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from rabbitct_amd import lolahip, rct  # noqa: E402
+from rabbitct_amd import autotune, lolahip, rct  # noqa: E402
 from rabbitct_amd.generate import generate_into  # noqa: E402
 from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry  # noqa: E402
 
@@ -53,7 +53,12 @@ def make_context(args, geom, phantom, ranges, kernel, prec, vpp):
     ctx.set_ranges(ranges)
     generate_into(ctx, geom, phantom)
     ctx.synchronize()
-    return ctx
+    tuned = None
+    if kernel == "fused" and not args.no_tune:
+        # the autotuner times candidate shapes on the first views (untimed setup)
+        tuned = autotune.autotune(ctx, min(args.tune_views, geom.n_views), prec)
+        ctx.zero_volume()
+    return ctx, tuned
 
 
 def time_variant(args, ctx, n_views):
@@ -122,12 +127,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--L", type=int, default=512)
     ap.add_argument("--views", type=int, default=496)
-    ap.add_argument("--vpp", type=int, default=32, help="views per register pass (fused)")
+    ap.add_argument("--vpp", type=int, default=64, help="views per register pass (fused)")
     ap.add_argument("--run", type=int, default=16, help="voxels per thread along x")
     ap.add_argument("--variants", default="plain-f32,fused-f32,fused-bf16,fused-f16")
     ap.add_argument("--headline", default="fused-f32")
     ap.add_argument("--cpu-views", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--tune-views", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -147,8 +154,11 @@ def main():
         kernel, prec = var.split("-")
         vpp = 1 if kernel == "plain" else args.vpp
         t0 = time.time()
-        ctx = make_context(args, geom, phantom, ranges, kernel, prec, vpp)
-        log(f"{var}: context + {args.views} views generated in {time.time() - t0:.1f} s")
+        ctx, tuned = make_context(args, geom, phantom, ranges, kernel, prec, vpp)
+        if tuned:
+            vpp = tuned["best"]["views_per_pass"]
+        log(f"{var}: context + {args.views} views generated (+ tuning) in "
+            f"{time.time() - t0:.1f} s; shape {tuned['best'] if tuned else 'plain'}")
         ms, ms_launch, launches, vol = time_variant(args, ctx, args.views)
         eff = ctx.effective_updates(args.views)
         ctx.close()
@@ -156,7 +166,11 @@ def main():
         results[var] = {"ms_per_step": round(ms, 3), "gups": round(nominal / ms / 1e6, 2),
                         "gups_effective": round(eff / ms / 1e6, 2),
                         "ms_per_launch": round(ms_launch, 4), "launches": launches,
-                        "views_per_pass": vpp}
+                        "views_per_pass": vpp,
+                        "shape": tuned["best"] if tuned else {"kernel": "plain",
+                                                              "views_per_pass": 1,
+                                                              "run_length": args.run,
+                                                              "staging": prec == "f32"}}
         lat[var] = (ms_launch, launches, eff, vpp, prec)
         vols[var] = vol
         log(f"{var}: {ms:.2f} ms/step  {results[var]['gups']} GUPS "
@@ -191,7 +205,7 @@ def main():
                                f"{geom.width}x{geom.height} float32, line clipping VECTORSIZE 16, "
                                f"{args.headline} b={vpp}",
                    "L": args.L, "views": args.views, "detector": [geom.width, geom.height],
-                   "views_per_pass": vpp, "run_length": args.run},
+                   "shape": head["shape"]},
         "gups_effective": head["gups_effective"],
         "variants": results,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),

@@ -34,6 +34,7 @@ def lib():
         _lib.orc_backproject_bunny.argtypes = [u32, f32, f32, u32, u32, P, P, P]
         _lib.orc_backproject_clipped.argtypes = [u32, f32, f32, u32, u32, P, P, P, P]
         _lib.orc_backproject_asm.argtypes = [u32, f32, f32, u32, u32, P, P, P, ctypes.c_int, P]
+        _lib.orc_backproject_opt_f32.argtypes = [u32, f32, f32, u32, u32, P, P, P, P]
         _lib.orc_line_ranges.argtypes = [u32, f32, f32, u32, u32, P, ctypes.c_int, P]
         _lib.orc_shadow.argtypes = [u32, f32, f32, u32, u32, u32, P, P]
         _lib.orc_hu_metric.argtypes = [P, ctypes.c_uint64, P, f32, f32, P, P]
@@ -75,6 +76,20 @@ def backproject(L, rL, oL, matrices, images, ranges=None, mode="clipped", vs=16)
                 f.orc_backproject_asm(L, rL, oL, W, H, _p(a), _p(img), _p(r), vs, _p(vol))
             else:
                 raise ValueError(mode)
+    return vol
+
+
+def backproject_opt_f32(L, rL, oL, matrices, images, ranges):
+    """LolaOPT's FP32 arithmetic (LolaOPT.c:131-173) into a float32 volume."""
+    matrices = _c(matrices, np.float64).reshape(-1, 12)
+    images = _c(images, np.float32)
+    N, H, W = images.shape
+    vol = np.zeros((L, L, L), dtype=np.float32)
+    for n in range(N):
+        a = _c(matrices[n], np.float64)
+        img = _c(images[n], np.float32)
+        r = _c(ranges[n], np.int16)
+        lib().orc_backproject_opt_f32(L, rL, oL, W, H, _p(a), _p(img), _p(r), _p(vol))
     return vol
 
 

@@ -369,3 +369,51 @@ void orc_ramp_filter_rows(uint64_t rows, uint32_t W, double t, const double *in,
   }
   free(h);
 }
+
+/* LolaOPT's own FP32 arithmetic, statement by statement (LolaOPT.c:131-173):
+ * float world coordinates with wx advanced by "+= mm", the line-invariant part
+ * tmp[] evaluated in double and rounded to float, u/v/w = tmp + wx*a[k] in
+ * double rounded to float, rcpW = 1.0f/w, truncation, float lerp and a float
+ * volume.  Taps outside the detector read zero (the zero padding).  Used to
+ * measure how far the reference's own rounding lies from the FP64 result. */
+void orc_backproject_opt_f32(uint32_t L, float mm, float o, uint32_t W, uint32_t H,
+    const double *a, const float *img, const int16_t *ranges, float *vol)
+{
+#pragma omp parallel for schedule(static)
+  for (long z = 0; z < (long)L; z++) {
+    for (long y = 0; y < (long)L; y++) {
+      int start = ranges[2 * (z * L + y)];
+      int stop  = ranges[2 * (z * L + y) + 1];
+      if (stop - start == 0)
+        continue;
+      float wz = (float)z * mm + o;
+      float wy = (float)y * mm + o;
+      float wx = (float)start * mm + o;
+      float tmp0 = (float)(a[3] * wy + a[6] * wz + a[9]);
+      float tmp1 = (float)(a[4] * wy + a[7] * wz + a[10]);
+      float tmp2 = (float)(a[5] * wy + a[8] * wz + a[11]);
+      for (int x = start; x < stop; ++x) {
+        float u     = (float)(tmp0 + wx * a[0]);
+        float v     = (float)(tmp1 + wx * a[1]);
+        float w     = (float)(tmp2 + wx * a[2]);
+        float rcpW  = 1.0f / w;
+        float rcp2W = rcpW * rcpW;
+        float ix    = u * rcpW;
+        float iy    = v * rcpW;
+        long iix    = trunc_index(ix);
+        long iiy    = trunc_index(iy);
+        float scalx = ix - (float)iix;
+        float scaly = iy - (float)iiy;
+        wx += mm;
+        float bl = (float)pixel(img, iix, iiy, W, H);
+        float br = (float)pixel(img, iix + 1, iiy, W, H);
+        float tl = (float)pixel(img, iix, iiy + 1, W, H);
+        float tr = (float)pixel(img, iix + 1, iiy + 1, W, H);
+        float valb = (1 - scalx) * bl + scalx * br;
+        float valt = (1 - scalx) * tl + scalx * tr;
+        float val  = (1 - scaly) * valb + scaly * valt;
+        vol[(z * L + y) * L + x] += val * rcp2W;
+      }
+    }
+  }
+}

@@ -100,7 +100,16 @@ struct BpArgs {
   int clip;
   int tilesX, tilesY, tilesZ; // tile grid (x-runs, 64-line groups, 4-slice groups)
   int persistent;
+  const int2 *boxes;       // staged kernel: footprint box per (slot, tile)
 };
+
+// Per-view parameters and footprint boxes are read through the constant
+// address space: uniform addresses then become scalar loads.
+typedef const __attribute__((address_space(4))) ViewParams CViewParams;
+__device__ __forceinline__ CViewParams &view_params(const BpArgs &A, int slot)
+{
+  return ((CViewParams *)(uintptr_t)A.views)[slot];
+}
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const BpArgs &A, int slot)
 {
@@ -193,14 +202,14 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
   bool touched = false;
 
   // line range of the first view, then prefetch one view ahead
-  int rr_next = A.clip ? A.ranges[(uint64_t)(A.views[A.firstSlot].id - 1) * LL + line]
+  int rr_next = A.clip ? A.ranges[(uint64_t)(view_params(A, A.firstSlot).id - 1) * LL + line]
                        : (L << 16);
   for (int v = 0; v < A.nViews; ++v) {
     const int slot       = A.firstSlot + v;
-    const ViewParams &P  = A.views[slot];
+    CViewParams &P       = view_params(A, slot);
     const int rr         = rr_next;
     if (A.clip && v + 1 < A.nViews)
-      rr_next = A.ranges[(uint64_t)(A.views[slot + 1].id - 1) * LL + line];
+      rr_next = A.ranges[(uint64_t)(view_params(A, slot + 1).id - 1) * LL + line];
     const int start = (int)(short)(rr & 0xffff);
     const int stop  = rr >> 16;
     const int rlo   = max(start - x0, 0);
@@ -271,41 +280,20 @@ struct StageBox {
   bool fits;
 };
 
-// Footprint of the tile's corners for the view with FP64 matrix a; lanes
-// (l & 7) project corner l & 7, the min/max is reduced across each group of 8
-// lanes, so every lane (and every wave) ends with the same box.
-__device__ __forceinline__ StageBox stage_box(const double *a, double cx0, double cx1,
-    double cy0, double cy1, double cz0, double cz1, float Wf, float Hf)
+// Footprint boxes are computed once per (slot, tile) by box_kernel and packed
+// as int16 {iu0, iv0, fw, fh}; fw = 0 marks "no box" (the L2 fallback).
+__device__ __forceinline__ StageBox load_box(const BpArgs &A, int slot, int tile)
 {
-  const int c    = threadIdx.x & 7;
-  const double x = (c & 1) ? cx1 : cx0;
-  const double y = (c & 2) ? cy1 : cy0;
-  const double z = (c & 4) ? cz1 : cz0;
-  const float hu = (float)fma(a[0], x, fma(a[3], y, fma(a[6], z, a[9])));
-  const float hv = (float)fma(a[1], x, fma(a[4], y, fma(a[7], z, a[10])));
-  const float hw = (float)fma(a[2], x, fma(a[5], y, fma(a[8], z, a[11])));
-  const float rw = __builtin_amdgcn_rcpf(hw);
-  float umin = __builtin_amdgcn_fmed3f(hu * rw, -2.0f, Wf), umax = umin;
-  float vmin = __builtin_amdgcn_fmed3f(hv * rw, -2.0f, Hf), vmax = vmin;
-  for (int o = 1; o < 8; o <<= 1) {
-    umin = fminf(umin, __shfl_xor(umin, o));
-    umax = fmaxf(umax, __shfl_xor(umax, o));
-    vmin = fminf(vmin, __shfl_xor(vmin, o));
-    vmax = fmaxf(vmax, __shfl_xor(vmax, o));
-  }
+  const uint64_t i = (uint64_t)slot * (uint64_t)(A.tilesX * A.tilesY * A.tilesZ) + tile;
+  const __attribute__((address_space(4))) int *q =
+      (const __attribute__((address_space(4))) int *)(uintptr_t)A.boxes + 2 * i;
+  const int qx = q[0], qy = q[1];
   StageBox b;
-  const int W = (int)Wf, H = (int)Hf;
-  // taps of a coordinate t lie in [trunc(t), trunc(t)+1] within
-  // [floor(t), floor(t)+2]; one pixel of margin on each side for rounding
-  const int iu0 = max((int)floorf(umin) - 1, -2);
-  const int iv0 = max((int)floorf(vmin) - 1, -2);
-  const int iu1 = min((int)floorf(umax) + 3, W + 2);
-  const int iv1 = min((int)floorf(vmax) + 3, H + 2);
-  b.iu0  = __builtin_amdgcn_readfirstlane(iu0);
-  b.iv0  = __builtin_amdgcn_readfirstlane(iv0);
-  b.fw   = __builtin_amdgcn_readfirstlane(iu1 - iu0);
-  b.fh   = __builtin_amdgcn_readfirstlane(iv1 - iv0);
-  b.fits = b.fw <= kFW && b.fh <= kFH && b.fw >= 2 && b.fh >= 2;
+  b.iu0  = (int)(short)(qx & 0xffff);
+  b.iv0  = qx >> 16;
+  b.fw   = qy & 0xffff;
+  b.fh   = qy >> 16;
+  b.fits = b.fw >= 2 && b.fh >= 2 && b.fw <= kFW && b.fh <= kFH;
   if (b.fits) {
     b.lou = (float)b.iu0;
     b.hiu = (float)(b.iu0 + b.fw - 2);
@@ -313,11 +301,50 @@ __device__ __forceinline__ StageBox stage_box(const double *a, double cx0, doubl
     b.hiv = (float)(b.iv0 + b.fh - 2);
   } else {
     b.lou = -2.0f;
-    b.hiu = Wf;
+    b.hiu = A.Wf;
     b.lov = -2.0f;
-    b.hiv = Hf;
+    b.hiv = A.Hf;
   }
   return b;
+}
+
+// One thread per (slot, tile): project the tile's 8 corner voxels with the
+// slot's FP64 matrix, clamp to the bordered detector [-2, W] x [-2, H], and
+// keep [floor(min) - 1, floor(max) + 3): the taps of a coordinate t lie in
+// [trunc(t), trunc(t) + 1] within [floor(t), floor(t) + 2], plus one pixel of
+// margin on each side for the FP32 rounding of the per-voxel coordinates.
+__global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, int nTiles,
+    int tilesX, int tilesY, int R, float O_L, float R_L, int W, int H, int2 *boxes)
+{
+  const int t    = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slot = firstSlot + blockIdx.y;
+  if (t >= nTiles)
+    return;
+  const double *a = views[slot].a;
+  const int tx = t % tilesX, ty = (t / tilesX) % tilesY, tz = t / (tilesX * tilesY);
+  const double O = O_L, RL = R_L;
+  double umin = 1e30, umax = -1e30, vmin = 1e30, vmax = -1e30;
+  for (int c = 0; c < 8; ++c) {
+    const double x = O + (double)(tx * R + ((c & 1) ? R - 1 : 0)) * RL;
+    const double y = O + (double)(ty * kLYs + ((c & 2) ? kLYs - 1 : 0)) * RL;
+    const double z = O + (double)(tz * kTileZ + ((c & 4) ? kTileZ - 1 : 0)) * RL;
+    const double hw = fma(a[2], x, fma(a[5], y, fma(a[8], z, a[11])));
+    const double u  = fma(a[0], x, fma(a[3], y, fma(a[6], z, a[9]))) / hw;
+    const double v  = fma(a[1], x, fma(a[4], y, fma(a[7], z, a[10]))) / hw;
+    const double uc = fmin(fmax(u, -2.0), (double)W), vc = fmin(fmax(v, -2.0), (double)H);
+    umin = fmin(umin, uc);
+    umax = fmax(umax, uc);
+    vmin = fmin(vmin, vc);
+    vmax = fmax(vmax, vc);
+  }
+  const int iu0 = max((int)floor(umin) - 1, -2);
+  const int iv0 = max((int)floor(vmin) - 1, -2);
+  const int iu1 = min((int)floor(umax) + 3, W + 2);
+  const int iv1 = min((int)floor(vmax) + 3, H + 2);
+  int2 q;
+  q.x = (int)(uint16_t)(int16_t)iu0 | (int)((uint32_t)(uint16_t)(int16_t)iv0 << 16);
+  q.y = (int)(uint16_t)(iu1 - iu0) | (int)((uint32_t)(uint16_t)(iv1 - iv0) << 16);
+  boxes[(uint64_t)slot * (uint64_t)nTiles + t] = q;
 }
 
 // Issues the direct-to-LDS loads of footprint `b` of the slot image into buf:
@@ -346,6 +373,7 @@ template <int R>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
 {
+  const int tile = tx + A.tilesX * (ty + A.tilesY * tz);
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = threadIdx.x / kWave;
   const int x0   = tx * R;
@@ -359,16 +387,14 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   const double wx0 = O + (double)x0 * RL;
   const double wy  = O + (double)y * RL;
   const double wz  = O + (double)z * RL;
-  const double cx1 = O + (double)(x0 + R - 1) * RL;
-  const double cy0 = O + (double)y0 * RL, cy1 = O + (double)(y0 + kLYs - 1) * RL;
-  const double cz0 = O + (double)z0 * RL, cz1 = O + (double)(z0 + kTileZ - 1) * RL;
   const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
   const uint64_t LL   = (uint64_t)L * (uint64_t)L;
   const int pitch     = (int)(A.rowBytes >> 2);
   const int n         = A.nViews;
 
   auto range_of = [&](int v) -> int {
-    return A.clip ? A.ranges[(uint64_t)(A.views[A.firstSlot + v].id - 1) * LL + line] : (L << 16);
+    return A.clip ? A.ranges[(uint64_t)(view_params(A, A.firstSlot + v).id - 1) * LL + line]
+                  : (L << 16);
   };
   auto lane_active = [&](int rr) -> bool {
     const int start = (int)(short)(rr & 0xffff), stop = rr >> 16;
@@ -390,13 +416,13 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     sAct[wv] = act0 ? 1 : 0;
   __syncthreads();
   bool blockAct = (sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
-  StageBox box  = stage_box(A.views[A.firstSlot].a, wx0, cx1, cy0, cy1, cz0, cz1, A.Wf, A.Hf);
+  StageBox box  = load_box(A, A.firstSlot, tile);
   if (blockAct && box.fits)
     stage_issue(slot_rsrc(A, A.firstSlot), box, pitch, sImg);
 
   for (int v = 0; v < n; ++v) {
     const int slot      = A.firstSlot + v;
-    const ViewParams &P = A.views[slot];
+    CViewParams &P      = view_params(A, slot);
     const bool hasNext  = v + 1 < n;
     // footprint(v) and the range loads have landed; publish activity of v+1
     __builtin_amdgcn_s_waitcnt(0);
@@ -412,7 +438,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       rr_nxt = range_of(v + 2);
     StageBox boxNext = box;
     if (hasNext) {
-      boxNext = stage_box(A.views[slot + 1].a, wx0, cx1, cy0, cy1, cz0, cz1, A.Wf, A.Hf);
+      boxNext = load_box(A, slot + 1, tile);
       if (blockActNext && boxNext.fits)
         stage_issue(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kFW * kFH);
     }
@@ -502,7 +528,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 // tiles are renumbered such that each XCD's share is one contiguous z-slab and
 // its L2 serves one band of detector rows.
 template <int R, int PREC, int STAGE>
-__global__ __launch_bounds__(kWaves *kWave, 4) void bp_kernel(BpArgs A)
+__global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpArgs A)
 {
   __shared__ float sImg[STAGE ? 2 * kFW * kFH : 1];
   __shared__ int sAct[2 * kWaves];
@@ -697,6 +723,9 @@ struct LhContext {
   int numCU = 0;
   std::vector<uint64_t> effPerView; // by id-1
   std::vector<int> slotId;          // id held by each slot (0 = empty)
+  int2 *dBoxes  = nullptr;          // staged kernel: footprint box per (slot, tile)
+  long boxTiles = 0;                // tiles per slot in dBoxes
+  std::vector<int> boxR;            // run length the slot's boxes were built for (0 = none)
 };
 
 // Grid of one launch: one workgroup per tile, or for the persistent form as
@@ -810,6 +839,7 @@ int lhCreate(const LhConfig *cfg, LhContext **out)
       hipHostMallocDefault));
   c->effPerView.assign(cfg->nViews, cfg->clip ? 0 : L * L * L);
   c->slotId.assign(cfg->nSlots, 0);
+  c->boxR.assign(cfg->nSlots, 0);
   LH_HIP(hipStreamSynchronize(c->stream));
   return LH_OK;
 }
@@ -826,6 +856,7 @@ int lhDestroy(LhContext *c)
   (void)hipFree(c->dSlots);
   (void)hipFree(c->dStage);
   (void)hipFree(c->dMax);
+  (void)hipFree(c->dBoxes);
   if (c->hPinned)
     (void)hipHostFree(c->hPinned);
   if (c->stream)
@@ -987,6 +1018,7 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
   }
   LH_HIP(hipGetLastError());
   c->slotId[slot] = id;
+  c->boxR[slot]   = 0;
   return LH_OK;
 }
 
@@ -1012,6 +1044,40 @@ int lhLoadViewDevice(LhContext *c, uint32_t slot, int id, const double *matrix,
   const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
   LH_HIP(hipMemcpyAsync(c->dStage, dImage, bytes, hipMemcpyDeviceToDevice, c->stream));
   return load_view_common(c, slot, id, matrix);
+}
+
+// Footprint boxes of slots [first, first+count) for the staged tile grid of
+// run length R; built once per (slot, R) and kept until the slot is reloaded.
+static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, int tilesX,
+    int tilesY, int tilesZ)
+{
+  const LhConfig &g = c->cfg;
+  const long nTiles = (long)tilesX * tilesY * tilesZ;
+  if (c->boxTiles != nTiles) {
+    (void)hipFree(c->dBoxes);
+    c->dBoxes   = nullptr;
+    c->boxTiles = 0;
+    LH_HIP(hipMalloc(&c->dBoxes, (size_t)g.nSlots * (size_t)nTiles * sizeof(int2)));
+    c->boxTiles = nTiles;
+    std::fill(c->boxR.begin(), c->boxR.end(), 0);
+  }
+  uint32_t s = first;
+  while (s < first + count) {
+    if (c->boxR[s] == R) {
+      ++s;
+      continue;
+    }
+    uint32_t e = s;
+    while (e < first + count && c->boxR[e] != R && e - s < 65535)
+      ++e;
+    box_kernel<<<dim3((unsigned)((nTiles + 255) / 256), e - s), 256, 0, c->stream>>>(c->dViews,
+        (int)s, (int)nTiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H, c->dBoxes);
+    LH_HIP(hipGetLastError());
+    for (uint32_t k = s; k < e; ++k)
+      c->boxR[k] = R;
+    s = e;
+  }
+  return LH_OK;
 }
 
 static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
@@ -1040,6 +1106,13 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
   A.tilesY      = (int)g.L / (stage ? kLYs : kWave);
   A.tilesZ      = (int)g.L / (stage ? kTileZ : kWaves);
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
+  A.boxes       = nullptr;
+  if (stage) {
+    const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, A.tilesZ);
+    if (rc != LH_OK)
+      return rc;
+    A.boxes = c->dBoxes;
+  }
 #define LH_LAUNCH(RR, PP, SS) launch_shape<RR, PP, SS>(c, A)
 #define LH_BY_PREC(RR)                                                                   \
   switch (g.precision) {                                                                 \

@@ -156,17 +156,24 @@ def _full_detector_case(L2, n_views, seed, noise):
     return geom, imgs, ranges, ref
 
 
-@pytest.mark.parametrize("L2,staging", [(256, True), (256, False), (512, True)])
+@pytest.mark.parametrize("L2,staging", [(256, True), (256, False), (512, True), (512, False)])
 def test_full_detector_phantom_vs_fp64(L2, staging):
     """Full 1248x960 detector, phantom projections: at L=256 the staged kernel
     mixes LDS footprints with the L2 fallback (footprints up to 237 px wide),
-    at L=512 every footprint fits."""
+    at L=512 every footprint fits.  With 3 views the reference's own FP32
+    arithmetic (LolaOPT.c:131-173, restated in the oracle) is itself about
+    1e-4 of the range away from FP64; the bound is 1e-4 or 1.5x the
+    reference's distance on the same input, whichever is larger."""
     geom, imgs, ranges, ref = _full_detector_case(L2, 3, 17, noise=False)
     v = lolahip.backproject_views(L2, geom.matrices, imgs, ranges, staging=staging)
+    opt = orc.backproject_opt_f32(L2, voxel_size(L2), volume_origin(L2), geom.matrices, imgs,
+                                  ranges)
     rngv = float(ref.max() - ref.min())
     err = np.abs(v - ref).max() / rngv
-    print(f"\nL={L2} staging={staging} phantom: max|d|/range {err:.2e}")
-    assert err <= TOL
+    err_ref = np.abs(opt - ref).max() / rngv
+    print(f"\nL={L2} staging={staging} phantom: max|d|/range {err:.2e} "
+          f"(LolaOPT arithmetic {err_ref:.2e})")
+    assert err <= max(TOL, 1.5 * err_ref)
 
 
 @pytest.mark.parametrize("staging", [True, False])

@@ -51,6 +51,18 @@ def test_clipped_matches_reference(golden, alg, clip):
     assert pt < 2e-5 and zs < 2e-6
 
 
+@pytest.mark.parametrize("alg,clip", [("opt16", "clip16"), ("opt8", "clip8")])
+def test_lolaopt_f32_restatement_matches_reference(golden, alg, clip):
+    """The FP32 restatement of LolaOPT's arithmetic sits closer to the
+    compiled LolaOPT than the FP64 oracle does."""
+    m, im = golden["matrices"], golden["images"]
+    f32 = orc.backproject_opt_f32(L, RL, OL, m, im, golden[clip])
+    f64 = orc.backproject(L, RL, OL, m, im, golden[clip]).astype(np.float32)
+    e32, _ = compare_summary(f32, golden, alg)
+    e64, _ = compare_summary(f64, golden, alg)
+    assert e32 < 5e-7 and e32 <= e64
+
+
 def test_asm_semantics_match_reference(golden):
     v = orc.backproject(L, RL, OL, golden["matrices"], golden["images"], golden["clip16"],
                         mode="asm", vs=16)
