@@ -120,6 +120,21 @@ def cpu_baseline(args, geom, phantom, ranges):
                               f"{out['LolaOPT']['seconds']:.3f} s"}
 
 
+def measured_traffic(variant, shape, L):
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), already scaled by the
+    counters' reading of a 1 GiB copy; None when no pass matches this shape."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    key = (f"{variant} L={L} vpp={shape['views_per_pass']} R={shape['run_length']} "
+           f"st={int(shape['staging'])}")
+    rec = data.get(key)
+    return None if rec is None else rec["bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +149,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a device-to-device copy of this size (HBM counter scale)")
     ap.add_argument("--tune-views", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -161,16 +178,20 @@ def main():
             f"{time.time() - t0:.1f} s; shape {tuned['best'] if tuned else 'plain'}")
         ms, ms_launch, launches, vol = time_variant(args, ctx, args.views)
         eff = ctx.effective_updates(args.views)
+        if args.calib_copy_mib:
+            cms = ctx.copy_calibration(args.calib_copy_mib << 20)
+            log(f"calibration copy {args.calib_copy_mib} MiB: {cms:.3f} ms "
+                f"({2 * (args.calib_copy_mib << 20) / cms / 1e6:.0f} GB/s read+write)")
         ctx.close()
         nominal = args.L ** 3 * args.views
         results[var] = {"ms_per_step": round(ms, 3), "gups": round(nominal / ms / 1e6, 2),
                         "gups_effective": round(eff / ms / 1e6, 2),
                         "ms_per_launch": round(ms_launch, 4), "launches": launches,
                         "views_per_pass": vpp,
-                        "shape": tuned["best"] if tuned else {"kernel": "plain",
-                                                              "views_per_pass": 1,
+                        "shape": tuned["best"] if tuned else {"kernel": kernel,
+                                                              "views_per_pass": vpp,
                                                               "run_length": args.run,
-                                                              "staging": prec == "f32"}}
+                                                              "staging": True}}
         lat[var] = (ms_launch, launches, eff, vpp, prec)
         vols[var] = vol
         log(f"{var}: {ms:.2f} ms/step  {results[var]['gups']} GUPS "
@@ -187,6 +208,7 @@ def main():
     bytes_per_update = 4 * s_p + 8.0 / vpp
     achieved = (eff / launches) * bytes_per_update / (ms_launch * 1e-3)
     head = results[args.headline]
+    traffic = measured_traffic(args.headline, head["shape"], args.L)
     line = {
         "metric": METRIC,
         "value": head["gups"],
@@ -210,7 +232,7 @@ def main():
         "variants": results,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "bytes_per_update": bytes_per_update,
                      "updates_per_launch": int(eff / launches)},
         "cpu_baseline": None,

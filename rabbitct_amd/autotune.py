@@ -17,7 +17,7 @@ PASSES = (16, 32, 64)
 
 
 def candidates(precision: str):
-    stagings = (True, False) if precision == "f32" else (False,)
+    stagings = (True, False)
     return [("fused", vpp, run, st) for st, run, vpp in itertools.product(stagings, RUNS, PASSES)]
 
 

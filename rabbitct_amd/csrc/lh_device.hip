@@ -268,6 +268,9 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
 // LDS index is in bounds by construction (for voxels inside the tile the
 // clamp is the identity).  A footprint larger than kFW x kFH falls back to
 // L2 gathers for that view.
+#ifndef LH_FENCE
+#define LH_FENCE 4 // updates per scheduling group in the staged loop
+#endif
 constexpr int kFW    = 128; // LDS row pitch (floats); ds_read2 offsets stay <= 255
 constexpr int kFH    = 39;  // LDS rows per buffer (2 buffers x 39 x 128 x 4 B: 4 workgroups per CU)
 constexpr int kLYs   = 32;  // lanes along y in the staged tile
@@ -353,26 +356,30 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
 // beyond fw are copied too and never read; past the end of the slot the
 // buffer bounds return zero.  Completion is awaited by the caller
 // (s_waitcnt vmcnt(0) and a barrier).
+// elem = bytes per slot element: 4 (FP32 image) or 8 (half "quad" image, of
+// which the first dword, the horizontal pair p(i,j), p(i+1,j), is copied).
+template <int ELEM>
 __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StageBox &b,
     int pitch, float *buf)
 {
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = threadIdx.x / kWave;
-  const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * 4u;
+  const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * ELEM;
   for (int r = wv; r < b.fh; r += kWaves) {
-    const uint32_t row = base + (uint32_t)(r * pitch) * 4u;
+    const uint32_t row = base + (uint32_t)(r * pitch) * ELEM;
     for (int c0 = 0; c0 < b.fw; c0 += kWave) {
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
           (__attribute__((address_space(3))) void *)(buf + r * kFW + c0), 4,
-          (int)(row + (uint32_t)(c0 + lane) * 4u), 0, 0, 0);
+          (int)(row + (uint32_t)(c0 + lane) * ELEM), 0, 0, 0);
     }
   }
 }
 
-template <int R>
+template <int R, int PREC>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
 {
+  constexpr int ELEM = PREC == LH_PREC_F32 ? 4 : 8;
   const int tile = tx + A.tilesX * (ty + A.tilesY * tz);
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = threadIdx.x / kWave;
@@ -389,7 +396,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   const double wz  = O + (double)z * RL;
   const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
   const uint64_t LL   = (uint64_t)L * (uint64_t)L;
-  const int pitch     = (int)(A.rowBytes >> 2);
+  const int pitch     = (int)(A.rowBytes / ELEM);
   const int n         = A.nViews;
 
   auto range_of = [&](int v) -> int {
@@ -418,7 +425,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   bool blockAct = (sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
   StageBox box  = load_box(A, A.firstSlot, tile);
   if (blockAct && box.fits)
-    stage_issue(slot_rsrc(A, A.firstSlot), box, pitch, sImg);
+    stage_issue<ELEM>(slot_rsrc(A, A.firstSlot), box, pitch, sImg);
 
   for (int v = 0; v < n; ++v) {
     const int slot      = A.firstSlot + v;
@@ -440,7 +447,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     if (hasNext) {
       boxNext = load_box(A, slot + 1, tile);
       if (blockActNext && boxNext.fits)
-        stage_issue(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kFW * kFH);
+        stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kFW * kFH);
     }
 
     const int start   = (int)(short)(rr & 0xffff);
@@ -465,12 +472,30 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
           const float fx = __builtin_truncf(ix);
           const float fy = __builtin_truncf(iy);
           // fy*kFW + fx is an exact integer in FP32 (|.| < 2^24): one conversion
-          const float *q  = sBase + (int)fmaf(fy, (float)kFW, fx);
-          const float p00 = q[0], p10 = q[1];
-          const float p01 = q[kFW], p11 = q[kFW + 1];
+          const int a    = (int)fmaf(fy, (float)kFW, fx);
           const float ax = ix - fx, ay = iy - fy;
-          const float b   = fmaf(ax, p10 - p00, p00);
-          const float t   = fmaf(ax, p11 - p01, p01);
+          float b, t;
+          if constexpr (PREC == LH_PREC_F32) {
+            const float *q  = sBase + a;
+            const float p00 = q[0], p10 = q[1];
+            const float p01 = q[kFW], p11 = q[kFW + 1];
+            b = fmaf(ax, p10 - p00, p00);
+            t = fmaf(ax, p11 - p01, p01);
+          } else {
+            // one dword per pixel: the horizontal pair (p(i,j), p(i+1,j))
+            const uint32_t *q = reinterpret_cast<const uint32_t *>(sBase) + a;
+            const uint32_t q0 = q[0], q1 = q[kFW];
+            if constexpr (PREC == LH_PREC_F16) {
+              // (1-ax) p0 + ax p1 with the FP16 operands widened inside the FMA
+              b = fmaf(ax, f16_hi(q0), fmaf(-ax, f16_lo(q0), f16_lo(q0)));
+              t = fmaf(ax, f16_hi(q1), fmaf(-ax, f16_lo(q1), f16_lo(q1)));
+            } else {
+              const float p00 = bf16_lo(q0), p10 = bf16_hi(q0);
+              const float p01 = bf16_lo(q1), p11 = bf16_hi(q1);
+              b = fmaf(ax, p10 - p00, p00);
+              t = fmaf(ax, p11 - p01, p01);
+            }
+          }
           return fmaf(ay, t - b, b) * (rw * rw);
         };
         // groups of four updates are fenced so the scheduler keeps at most four
@@ -479,7 +504,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             acc[r] += update(r);
-            if ((r & 3) == 3)
+            if ((r % LH_FENCE) == LH_FENCE - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         } else {
@@ -487,7 +512,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
           for (int r = 0; r < R; ++r) {
             if (r >= rlo && r < rhi)
               acc[r] += update(r);
-            if ((r & 3) == 3)
+            if ((r % LH_FENCE) == LH_FENCE - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         }
@@ -498,7 +523,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         for (int r = 0; r < R; ++r) {
           if (r >= rlo && r < rhi) {
             const float fr = (float)r;
-            acc[r] += voxel_update<LH_PREC_F32>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0),
+            acc[r] += voxel_update<PREC>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0),
                 fmaf(fr, dw, w0));
           }
           __builtin_amdgcn_sched_barrier(0);
@@ -534,7 +559,7 @@ __global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpA
   __shared__ int sAct[2 * kWaves];
   if (!A.persistent) {
     if constexpr (STAGE)
-      bp_tile_lds<R>(A, blockIdx.x, blockIdx.y, blockIdx.z, sImg, sAct);
+      bp_tile_lds<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z, sImg, sAct);
     else
       bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
     return;
@@ -553,7 +578,7 @@ __global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpA
     const int tz = t / (A.tilesX * A.tilesY);
     if constexpr (STAGE) {
       __syncthreads(); // the previous tile's last LDS reads are done
-      bp_tile_lds<R>(A, tx, ty, tz, sImg, sAct);
+      bp_tile_lds<R, PREC>(A, tx, ty, tz, sImg, sAct);
     } else {
       bp_tile<R, PREC>(A, tx, ty, tz);
     }
@@ -1101,7 +1126,7 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
   A.nViews      = (int)count;
   A.clip        = g.clip;
   const int R   = (int)g.runLength;
-  const int stage = (g.staging && g.precision == LH_PREC_F32) ? 1 : 0;
+  const int stage = g.staging ? 1 : 0;
   A.tilesX      = (int)g.L / R;
   A.tilesY      = (int)g.L / (stage ? kLYs : kWave);
   A.tilesZ      = (int)g.L / (stage ? kTileZ : kWaves);
@@ -1119,8 +1144,12 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
   case LH_PREC_F32:                                                                      \
     if (stage) LH_LAUNCH(RR, LH_PREC_F32, 1); else LH_LAUNCH(RR, LH_PREC_F32, 0);          \
     break;                                                                               \
-  case LH_PREC_BF16: LH_LAUNCH(RR, LH_PREC_BF16, 0); break;                               \
-  default: LH_LAUNCH(RR, LH_PREC_F16, 0); break;                                          \
+  case LH_PREC_BF16:                                                                     \
+    if (stage) LH_LAUNCH(RR, LH_PREC_BF16, 1); else LH_LAUNCH(RR, LH_PREC_BF16, 0);        \
+    break;                                                                               \
+  default:                                                                               \
+    if (stage) LH_LAUNCH(RR, LH_PREC_F16, 1); else LH_LAUNCH(RR, LH_PREC_F16, 0);          \
+    break;                                                                               \
   }
   switch (R) {
   case 8: LH_BY_PREC(8); break;

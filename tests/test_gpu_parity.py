@@ -60,16 +60,17 @@ def test_fp32_against_fp64_oracle_full_volume(golden):
     assert err.max() / rng <= TOL
 
 
+@pytest.mark.parametrize("staging", [True, False])
 @pytest.mark.parametrize("prec", ["bf16", "f16"])
-def test_half_storage_error_vs_fp64(golden_border, prec):
+def test_half_storage_error_vs_fp64(golden_border, prec, staging):
     g = golden_border
-    v = _bp(g, precision=prec)
+    v = _bp(g, precision=prec, staging=staging)
     ref = orc.backproject(L, voxel_size(L), volume_origin(L), g["matrices"], g["images"],
                           g["clip16"])
     rng = float(ref.max() - ref.min())
     rmse = np.sqrt(((v - ref) ** 2).mean()) / rng
     mx = np.abs(v - ref).max() / rng
-    print(f"\n{prec}: rmse {rmse:.2e} max {mx:.2e} of range vs FP64")
+    print(f"\n{prec} staging={staging}: rmse {rmse:.2e} max {mx:.2e} of range vs FP64")
     assert rmse <= 1e-3
 
 
@@ -154,6 +155,18 @@ def _full_detector_case(L2, n_views, seed, noise):
     ranges = lolahip.compute_line_ranges(L2, geom.width, geom.height, geom.matrices, 16)
     ref = orc.backproject(L2, voxel_size(L2), volume_origin(L2), geom.matrices, imgs, ranges)
     return geom, imgs, ranges, ref
+
+
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+def test_full_detector_half_staged_vs_fp64(prec):
+    """Half-precision storage on the full detector at L=512 (staged path):
+    RMSE <= 1e-3 of the range against FP64."""
+    geom, imgs, ranges, ref = _full_detector_case(512, 3, 23, noise=False)
+    v = lolahip.backproject_views(512, geom.matrices, imgs, ranges, precision=prec)
+    rngv = float(ref.max() - ref.min())
+    rmse = float(np.sqrt(((v - ref) ** 2).mean()) / rngv)
+    print(f"\nL=512 {prec} staged: rmse {rmse:.2e} max {np.abs(v - ref).max() / rngv:.2e}")
+    assert rmse <= 1e-3
 
 
 @pytest.mark.parametrize("L2,staging", [(256, True), (256, False), (512, True), (512, False)])
