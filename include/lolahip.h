@@ -212,6 +212,76 @@ int lhGenerateViews(LhContext *ctx, uint32_t firstSlot, uint32_t count, int firs
     const float *parker, uint32_t nEll, const double *ell, double tau, double dbeta,
     int border, float *hostImages);
 
+/* vol *= s on the context's stream (the weight of a view subset: a sparse
+ * scan of every k-th view is scaled by k). */
+int lhScaleVolume(LhContext *ctx, float s);
+
+/* Device pointer of the context's resident volume (L^3 floats, [z][y][x]);
+ * borrowed, valid until lhDestroy. */
+int lhVolumeDevice(LhContext *ctx, float **dVol);
+
+/* ---------------------------------------------------------------------
+ * Sparse-view + learned correction (no counterpart in the reference).
+ * All pointers named d* are device memory; calls are synchronous.
+ * ------------------------------------------------------------------- */
+
+/* Unweighted analytic line integrals of an ellipsoid phantom (the raw data
+ * of held-out views) on the detector subsampled by `stride`:
+ * dOut = count x ceil(H/stride) x ceil(W/stride) floats. */
+int lhLineIntegrals(int W, int H, int stride, uint32_t count, const double *src,
+    const double *minv, uint32_t nEll, const double *ell, float *dOut);
+
+/* Ray-marching forward projector of a volume (L^3, voxel size R mm, origin
+ * O mm) at `count` views (src count x 3, minv count x 9), detector
+ * subsampled by `stride`, step h mm, trilinear samples, midpoint rule. */
+int lhForwardProject(const float *dVol, int L, float R, float O, int W, int H, int stride,
+    uint32_t count, const double *src, const double *minv, float h, float *dOut);
+
+/* out = sqrt(sum (a-b)^2 / sum b^2) over n floats. */
+int lhRelativeResidual(const float *dA, const float *dB, uint64_t n, double *out);
+
+/* Gathers n patches of P^3 voxels from a stack of nVols volumes of L^3:
+ * spec (host) = n x (volume, z0, y0, x0). */
+int lhExtractPatches(const float *dVols, int nVols, int L, const int *spec, int n, int P,
+    float *dOut);
+
+/* 3-level 3D U-Net: 3x3x3 convolutions (BF16 weights and activations, FP32
+ * accumulation on MFMA), ReLU, 2x2x2 average pooling, nearest upsampling,
+ * channel concatenation, a 1x1x1 head and a residual connection:
+ *   out = in + head(dec1(cat(up(dec2(cat(up(bottom(pool(enc2(pool(enc1(in))))))),
+ *                                  enc2))), enc1))
+ * channels enc1 16, enc2 32, bottom 64, dec2 32, dec1 16.  Parameters are
+ * FP32 masters updated by Adam (beta 0.9/0.999, eps 1e-8). */
+typedef struct LhUnet LhUnet;
+int lhUnetCreate(LhUnet **out, uint64_t seed);
+int lhUnetDestroy(LhUnet *net);
+uint64_t lhUnetParamCount(void);
+/* Flat FP32 parameters: per conv layer W[cout][27][cin] then b[cout] (layers
+ * enc1a enc1b enc2a enc2b bottom_a bottom_b dec2a dec2b dec1a dec1b; enc1a
+ * has 16 input channels of which only channel 0 is fed), then head w[16], b. */
+int lhUnetGetParams(LhUnet *net, float *host);
+int lhUnetSetParams(LhUnet *net, const float *host);
+int lhUnetGetGrads(LhUnet *net, float *host);
+/* dOut = net(dIn) for n cubes of D^3 floats (D a multiple of 64). */
+int lhUnetForward(LhUnet *net, const float *dIn, float *dOut, int n, int D);
+/* MSE loss of net(dIn) against dTarget, its gradients, and with lr > 0 one
+ * Adam step. */
+int lhUnetTrainStep(LhUnet *net, const float *dIn, const float *dTarget, int n, int D, float lr,
+    float *loss);
+/* Single-convolution entries (tests): BF16 channels-last tensors, W FP32
+ * [cout][27][cin] (host).  backward = 1: backward-data pass (dx holds cout
+ * channels, dy receives cin). */
+int lhConv3dTest(const uint16_t *dx, int cin, int cout, const float *W, const float *b,
+    uint16_t *dy, int n, int D, int relu, int backward);
+int lhConv3dWgradTest(const uint16_t *dx, int cin, const uint16_t *dg, int cout, float *dW,
+    int n, int D);
+
+/* Device memory for callers without another allocator. */
+int lhDeviceAlloc(void **p, uint64_t bytes);
+int lhDeviceFree(void *p);
+int lhCopyToDevice(void *dDst, const void *src, uint64_t bytes);
+int lhCopyToHost(void *dst, const void *dSrc, uint64_t bytes);
+
 /* Last error message of the calling thread. */
 const char *lhLastError(void);
 

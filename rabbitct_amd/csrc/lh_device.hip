@@ -672,6 +672,25 @@ __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_
   }
 }
 
+// vol *= s (float4 body, scalar tail)
+__global__ void scale_kernel(float *__restrict__ vol, uint64_t n, float s)
+{
+  const uint64_t n4 = n / 4;
+  float4 *v4 = reinterpret_cast<float4 *>(vol);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    float4 q = v4[i];
+    q.x *= s;
+    q.y *= s;
+    q.z *= s;
+    q.w *= s;
+    v4[i] = q;
+  }
+  for (uint64_t i = 4 * n4 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    vol[i] *= s;
+}
+
 // ---- line ranges on the GPU ----------------------------------------------
 // Bit-identical form of analyseGeometry.c:207-281: one thread per (z,y) line
 // scans x from the left for the first hit and from the right for the last;
@@ -1001,6 +1020,26 @@ int lhGetVolume(LhContext *c, float *host)
   LH_HIP(hipSetDevice(c->cfg.device));
   LH_HIP(hipMemcpyAsync(host, c->dVol, L * L * L * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   LH_HIP(hipStreamSynchronize(c->stream));
+  return LH_OK;
+}
+
+int lhScaleVolume(LhContext *c, float s)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhScaleVolume: null context");
+  const uint64_t n = (uint64_t)c->cfg.L * c->cfg.L * c->cfg.L;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  scale_kernel<<<(unsigned)std::min<uint64_t>((n / 4 + 255) / 256, 8192), 256, 0, c->stream>>>(
+      c->dVol, n, s);
+  LH_HIP(hipGetLastError());
+  return LH_OK;
+}
+
+int lhVolumeDevice(LhContext *c, float **dVol)
+{
+  if (!c || !dVol)
+    return fail(LH_E_ARG, "lhVolumeDevice: null argument");
+  *dVol = c->dVol;
   return LH_OK;
 }
 

@@ -92,7 +92,75 @@ __global__ void ramp_kernel(int W, int H, double tau, double scale, int border,
   }
 }
 
+// Raw line integrals of the pixels (i*stride, j*stride), no weights.
+__global__ void raw_integrals_kernel(int Wd, int stride, const double *__restrict__ geo,
+    int nEll, const double *__restrict__ ell, float *__restrict__ out)
+{
+  __shared__ double sEll[64 * 16];
+  for (int i = threadIdx.x; i < nEll * 16; i += blockDim.x)
+    sEll[i] = ell[i];
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= Wd)
+    return;
+  const double *src = geo, *minv = geo + 3;
+  double d[3];
+  for (int r = 0; r < 3; ++r)
+    d[r] = minv[3 * r + 0] * (double)(i * stride) + minv[3 * r + 1] * (double)(j * stride) +
+           minv[3 * r + 2];
+  const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  d[0] /= nrm;
+  d[1] /= nrm;
+  d[2] /= nrm;
+  double acc = 0.0;
+  for (int k = 0; k < nEll; ++k)
+    acc += sEll[16 * k + 6] * chord(sEll + 16 * k, src, d);
+  out[(size_t)j * Wd + i] = (float)acc;
+}
+
 } // namespace
+
+// Unweighted line integrals of an ellipsoid phantom on a detector subsampled
+// by `stride` (pixels (i*stride, j*stride)), for `count` views; the result
+// (count x ceil(H/stride) x ceil(W/stride) float32) goes to device memory
+// dOut.  Ray geometry per view: source src[3], Minv[9] (row-major, the ray
+// through pixel (u,v) runs along Minv (u,v,1)).
+extern "C" int lhLineIntegrals(int W, int H, int stride, uint32_t count, const double *src,
+    const double *minv, uint32_t nEll, const double *ell, float *dOut)
+{
+  if (W < 1 || H < 1 || stride < 1 || !src || !minv || !ell || !dOut || nEll == 0 || nEll > 64)
+    return lh_fail(LH_E_ARG, "lhLineIntegrals: bad arguments");
+  const int Wd = (W + stride - 1) / stride, Hd = (H + stride - 1) / stride;
+  std::vector<double> geo((size_t)count * 12);
+  for (uint32_t v = 0; v < count; ++v) {
+    for (int k = 0; k < 3; ++k)
+      geo[v * 12 + k] = src[3 * v + k];
+    for (int k = 0; k < 9; ++k)
+      geo[v * 12 + 3 + k] = minv[9 * v + k];
+  }
+  double *dGeo = nullptr, *dEll = nullptr;
+  int rc = LH_OK;
+  hipError_t e = hipMalloc(&dGeo, geo.size() * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&dEll, (size_t)nEll * 16 * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemcpy(dGeo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dEll, ell, (size_t)nEll * 16 * sizeof(double), hipMemcpyHostToDevice);
+  for (uint32_t v = 0; e == hipSuccess && v < count; ++v) {
+    raw_integrals_kernel<<<dim3((Wd + 255) / 256, Hd), 256>>>(Wd, stride, dGeo + 12 * v,
+        (int)nEll, dEll, dOut + (size_t)v * Wd * Hd);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipDeviceSynchronize();
+  if (e != hipSuccess)
+    rc = lh_fail(LH_E_HIP, "lhLineIntegrals: %s", hipGetErrorString(e));
+  (void)hipFree(dGeo);
+  (void)hipFree(dEll);
+  return rc;
+}
 
 extern "C" int lhGenerateViews(LhContext *c, uint32_t firstSlot, uint32_t count, int firstId,
     const double *matrices, const double *src, const double *minv, const double *axis,

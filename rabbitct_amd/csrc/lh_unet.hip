@@ -1,0 +1,1032 @@
+// lh_unet.hip -- 3-level 3D U-Net for the sparse-view correction (no
+// counterpart in the reference; it learns the map from a 124-view to a
+// 496-view backprojection).
+//
+// Tensors are channels-last BF16, [voxel][ld] with a channel slice at an
+// offset, so the two concatenations of the decoder are channel slices of one
+// buffer and cost no copy.  Every 3x3x3 convolution (stride 1, zero padding 1)
+// is an implicit GEMM on v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+//   M = 16 output voxels per wave (consecutive x), N = output channels (16 per
+//   MFMA tile), K = 27 taps x input channels.
+// K is packed in 32-wide chunks so that each chunk reads one wave-uniform tap
+// (or one of two taps, chosen by a lane bit):
+//   section 1: for each tap, for each full block of 32 input channels;
+//   section 2 (input channels = 16 mod 32): the last 16 channels of taps
+//   (2p, 2p+1) share one chunk, p = 0..13 (tap 27 is zero).
+// Lane l holds A rows (voxel l%16) and B columns (channel l%16) for the 8
+// K entries 8*(l/16)..+7 of a chunk: 8 consecutive channels of one tap, one
+// 16-byte load each.  The backward-data pass is the same kernel with the taps
+// mirrored and input/output channels swapped in the weight packing.  The
+// weight gradient is a separate MFMA kernel that reduces over voxels (K)
+// from transposed rows staged in LDS.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "lolahip.h"
+#include "lh_internal.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// raw buffer resource word 3 for gfx950 (32-bit untyped access); a load
+// whose offset lies past num_records returns zero
+constexpr int kRsrcWord3 = 0x00020000;
+constexpr uint32_t kMissOffset = 0x80000000u;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+  uint32_t b = __float_as_uint(f);
+  if ((b & 0x7fffffffu) > 0x7f800000u)
+    return (uint16_t)((b >> 16) | 0x40u);
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
+// Number of 32-wide K chunks and the chunk/entry of (tap, ci) for C input
+// channels (C a multiple of 16).
+__host__ __device__ __forceinline__ int kchunks(int C) { return 27 * (C / 32) + ((C % 32) ? 14 : 0); }
+__host__ __device__ __forceinline__ int kpos(int C, int tap, int ci)
+{
+  const int F = C / 32;
+  if (ci < 32 * F)
+    return (tap * F + ci / 32) * 32 + ci % 32;
+  return (27 * F + tap / 2) * 32 + (tap & 1) * 16 + (ci - 32 * F);
+}
+
+struct ConvArgs {
+  const uint16_t *x;  // input [vox][ldx], channels [xofs, xofs+cin)
+  const uint16_t *w;  // packed weights [cout][kp] (bf16)
+  const float *bias;  // [cout] or null
+  uint16_t *y;        // output [vox][ldy], channels [yofs, yofs+cout)
+  int ldx, xofs, ldy, yofs;
+  int D, n;           // cube edge, samples
+  int cin, cout, kp;
+  int relu, accumulate;
+  int xBytes;         // bytes addressable from x (buffer range, < 2^31)
+};
+
+__device__ __forceinline__ int tap_offset(int tap, int D)
+{
+  return ((tap / 9 - 1) * D + ((tap / 3) % 3 - 1)) * D + (tap % 3 - 1);
+}
+
+// One wave: MT rows of 16 output voxels (x runs, consecutive y) x NT*16
+// output channels.  Each B (weight) operand feeds MT MFMAs, each A operand NT.
+template <int NT, int MT>
+__global__ __launch_bounds__(256) void conv3_kernel(ConvArgs a)
+{
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int D    = a.D;
+  const int xbN  = D / 16, ybN = D / MT;
+  if (wave >= a.n * D * ybN * xbN)
+    return;
+  const int xb = wave % xbN;
+  int rest     = wave / xbN;
+  const int yb = rest % ybN;
+  rest /= ybN;
+  const int z = rest % D, n = rest / D;
+  const int m = lane & 15, kg = lane >> 4;
+  const int x = xb * 16 + m, y0 = yb * MT;
+  // in-bounds masks of the 27 taps for this lane's voxel in each row
+  uint32_t mask[MT];
+#pragma unroll
+  for (int r = 0; r < MT; ++r)
+    mask[r] = 0;
+  for (int t = 0; t < 27; ++t) {
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const bool okzx = (unsigned)(z + dz) < (unsigned)D && (unsigned)(x + dx) < (unsigned)D;
+#pragma unroll
+    for (int r = 0; r < MT; ++r) {
+      const bool ok = okzx && (unsigned)(y0 + r + dy) < (unsigned)D;
+      mask[r] |= (ok ? 1u : 0u) << t;
+    }
+  }
+  const size_t v0 = (((size_t)n * D + z) * D + y0) * D + x;
+  const uint32_t rowStride = (uint32_t)D * a.ldx * 2u;     // bytes between y rows
+  const uint32_t xoff = (uint32_t)((v0 * a.ldx + a.xofs + 8 * kg) * 2u);
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc((void *)a.x, (short)0, a.xBytes, kRsrcWord3);
+  const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
+  const size_t wtile = (size_t)16 * a.kp;
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int r = 0; r < MT; ++r)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      acc[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int F = a.cin / 32;
+  // section 1: (tap, 32-channel block); out-of-volume taps read zeros
+  int chunk = 0;
+  for (int tap = 0; tap < 27; ++tap) {
+    const uint32_t off = xoff + (uint32_t)(tap_offset(tap, D) * a.ldx * 2);
+    for (int b = 0; b < F; ++b, ++chunk) {
+      bf16x8 A[MT];
+#pragma unroll
+      for (int r = 0; r < MT; ++r) {
+        const uint32_t o = ((mask[r] >> tap) & 1u) ? off + r * rowStride + 64u * b : kMissOffset;
+        A[r] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(xr, o, 0, 0));
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x8 B = __builtin_bit_cast(bf16x8,
+            *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], B, acc[r][t], 0, 0, 0);
+      }
+    }
+  }
+  // section 2: the last 16 channels of tap pairs (2p, 2p+1); tap 27 is zero
+  if (a.cin % 32) {
+    const int ci0 = 32 * F + 8 * (kg & 1) - 8 * kg; // xoff already holds +8*kg
+    for (int p = 0; p < 14; ++p, ++chunk) {
+      const int tap = 2 * p + (kg >> 1);
+      const uint32_t off = xoff + (uint32_t)((tap_offset(tap, D) * a.ldx + ci0) * 2);
+      bf16x8 A[MT];
+#pragma unroll
+      for (int r = 0; r < MT; ++r) {
+        const bool ok = tap < 27 && ((mask[r] >> tap) & 1u);
+        const uint32_t o = ok ? off + r * rowStride : kMissOffset;
+        A[r] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(xr, o, 0, 0));
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x8 B = __builtin_bit_cast(bf16x8,
+            *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], B, acc[r][t], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds D[voxel 4*kg + i of the row][channel 16t + m]
+  const size_t ob = (((size_t)n * D + z) * D + y0) * D + xb * 16 + 4 * kg;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = 16 * t + m;
+    const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < MT; ++r) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        uint16_t *dst = a.y + (ob + (size_t)r * D + i) * a.ldy + a.yofs + co;
+        float val = acc[r][t][i] + bias;
+        if (a.relu)
+          val = fmaxf(val, 0.0f);
+        if (a.accumulate)
+          val += bf2f(*dst);
+        *dst = f2bf(val);
+      }
+    }
+  }
+}
+
+// Weight gradient dW[co][tap][ci] += sum_v g[v][co] * x[v + off(tap)][ci].
+// Grid: (blocks over x-rows) x 9 (dz, dy).  Per 32-voxel x segment, LDS
+// holds g transposed (gT[co][v]) and the input row at the three x shifts,
+// transposed (xT[dx][ci][v]); every MFMA operand is one aligned 16-byte LDS
+// read (M = 16 output channels, N = 16 input channels, K = 32 voxels).
+// The 3*COT*CIT tiles are spread over the 4 waves; partial sums are added to
+// dW atomically.
+struct WgradArgs {
+  const uint16_t *x; int ldx, xofs;
+  const uint16_t *g; int ldg, gofs;
+  float *dw;           // [cout][27][cin]
+  int D, nrows;        // rows = n*D*D x-rows of D voxels
+  int rowsPerBlock;
+};
+
+constexpr int kSeg  = 32;
+constexpr int kSegP = 40; // padded LDS row (80 bytes)
+
+template <int COT, int CIT>
+__global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
+{
+  constexpr int CIN = 16 * CIT, COUT = 16 * COT;
+  constexpr int NTILES = 3 * COT * CIT, PER = (NTILES + 3) / 4;
+  __shared__ __align__(16) uint16_t gT[COUT * kSegP];
+  __shared__ __align__(16) uint16_t xT[3 * CIN * kSegP];
+  const int dz = blockIdx.y / 3, dy = blockIdx.y % 3;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  const int D = a.D;
+  f32x4 acc[PER];
+#pragma unroll
+  for (int s = 0; s < PER; ++s)
+    acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int r0 = blockIdx.x * a.rowsPerBlock;
+  const int r1 = min(r0 + a.rowsPerBlock, a.nrows);
+  for (int row = r0; row < r1; ++row) {
+    const int yy = row % D, z = (row / D) % D, n = row / (D * D);
+    const int sz = z + dz - 1, sy = yy + dy - 1;
+    const bool rowOk = (unsigned)sz < (unsigned)D && (unsigned)sy < (unsigned)D;
+    const size_t gBase = (((size_t)n * D + z) * D + yy) * D;
+    const size_t xBase = (((size_t)n * D + sz) * D + sy) * D;
+    for (int x0 = 0; x0 < D; x0 += kSeg) {
+      const int seg = min(kSeg, D - x0); // voxels of this segment; the rest of K is zero
+      __syncthreads();
+      for (int i = threadIdx.x; i < COUT * kSeg; i += 256) {
+        const int co = i % COUT, xv = i / COUT;
+        gT[co * kSegP + xv] = xv < seg ? a.g[(gBase + x0 + xv) * a.ldg + a.gofs + co] : (uint16_t)0;
+      }
+      for (int i = threadIdx.x; i < 3 * CIN * kSeg; i += 256) {
+        const int ci = i % CIN, rem = i / CIN;
+        const int xv = rem % kSeg, dx = rem / kSeg;
+        const int sx = x0 + xv + dx - 1;
+        uint16_t val = 0;
+        if (rowOk && xv < seg && (unsigned)sx < (unsigned)D)
+          val = a.x[(xBase + sx) * a.ldx + a.xofs + ci];
+        xT[(dx * CIN + ci) * kSegP + xv] = val;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < PER; ++s) {
+        const int t = wv + 4 * s;
+        if (t < NTILES) {
+          const int dx = t / (COT * CIT), rem = t % (COT * CIT);
+          const int cot = rem / CIT, cit = rem % CIT;
+          const bf16x8 A = __builtin_bit_cast(bf16x8,
+              *reinterpret_cast<const uint4 *>(gT + (cot * 16 + m) * kSegP + 8 * kg));
+          const bf16x8 B = __builtin_bit_cast(bf16x8,
+              *reinterpret_cast<const uint4 *>(xT + (dx * CIN + cit * 16 + m) * kSegP + 8 * kg));
+          acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[s], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int t = wv + 4 * s;
+    if (t < NTILES) {
+      const int dx = t / (COT * CIT), rem = t % (COT * CIT);
+      const int cot = rem / CIT, cit = rem % CIT;
+      const int tap = blockIdx.y * 3 + dx;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = cot * 16 + 4 * kg + i, ci = cit * 16 + m;
+        atomicAdd(a.dw + ((size_t)co * 27 + tap) * CIN + ci, acc[s][i]);
+      }
+    }
+  }
+}
+
+// bias gradient: db[c] += sum_v g[v][c]
+__global__ void bgrad_kernel(const uint16_t *g, int ldg, int gofs, int C, int nvox, float *db)
+{
+  const int c = threadIdx.x % C;
+  const int lanesPerC = blockDim.x / C;
+  const int sub = threadIdx.x / C;
+  float s = 0.0f;
+  for (int v = blockIdx.x * lanesPerC + sub; v < nvox; v += gridDim.x * lanesPerC)
+    s += bf2f(g[(size_t)v * ldg + gofs + c]);
+  atomicAdd(db + c, s);
+}
+
+// g *= (y > 0) over a channel slice
+__global__ void relu_bwd_kernel(uint16_t *g, const uint16_t *y, int ld, int ofs, int C, int nvox)
+{
+  const size_t n = (size_t)nvox * C;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t v = i / C, c = i % C;
+    const size_t p = v * ld + ofs + c;
+    if (!(bf2f(y[p]) > 0.0f))
+      g[p] = 0;
+  }
+}
+
+// 2x2x2 average pooling, channel slices in and out
+__global__ void pool_kernel(const uint16_t *x, int ldx, int xofs, uint16_t *y, int ldy, int yofs,
+    int C, int Do, int n)
+{
+  const size_t total = (size_t)n * Do * Do * Do * C;
+  const int Di = 2 * Do;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int c = i % C;
+    size_t v = i / C;
+    const int xo = v % Do, yo = (v / Do) % Do, zo = (v / (Do * Do)) % Do, b = v / ((size_t)Do * Do * Do);
+    float s = 0.0f;
+    for (int q = 0; q < 8; ++q) {
+      const size_t vi = (((size_t)b * Di + 2 * zo + (q >> 2)) * Di + 2 * yo + ((q >> 1) & 1)) * Di + 2 * xo + (q & 1);
+      s += bf2f(x[vi * ldx + xofs + c]);
+    }
+    y[v * ldy + yofs + c] = f2bf(0.125f * s);
+  }
+}
+
+// pooling backward: gx += gy(parent) / 8
+__global__ void pool_bwd_kernel(const uint16_t *gy, int ldy, int yofs, uint16_t *gx, int ldx,
+    int xofs, int C, int Di, int n)
+{
+  const size_t total = (size_t)n * Di * Di * Di * C;
+  const int Do = Di / 2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int c = i % C;
+    const size_t v = i / C;
+    const int xi = v % Di, yi = (v / Di) % Di, zi = (v / (Di * Di)) % Di, b = v / ((size_t)Di * Di * Di);
+    const size_t vo = (((size_t)b * Do + zi / 2) * Do + yi / 2) * Do + xi / 2;
+    uint16_t *dst = gx + v * ldx + xofs + c;
+    *dst = f2bf(bf2f(*dst) + 0.125f * bf2f(gy[vo * ldy + yofs + c]));
+  }
+}
+
+// nearest 2x upsampling into a channel slice
+__global__ void up_kernel(const uint16_t *x, int ldx, int xofs, uint16_t *y, int ldy, int yofs,
+    int C, int Di, int n)
+{
+  const int Do = 2 * Di;
+  const size_t total = (size_t)n * Do * Do * Do * C;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int c = i % C;
+    const size_t v = i / C;
+    const int xo = v % Do, yo = (v / Do) % Do, zo = (v / (Do * Do)) % Do, b = v / ((size_t)Do * Do * Do);
+    const size_t vi = (((size_t)b * Di + zo / 2) * Di + yo / 2) * Di + xo / 2;
+    y[v * ldy + yofs + c] = x[vi * ldx + xofs + c];
+  }
+}
+
+// upsampling backward: gx = sum of the 8 children
+__global__ void up_bwd_kernel(const uint16_t *gy, int ldy, int yofs, uint16_t *gx, int ldx, int xofs,
+    int C, int Di, int n)
+{
+  const size_t total = (size_t)n * Di * Di * Di * C;
+  const int Do = 2 * Di;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int c = i % C;
+    const size_t v = i / C;
+    const int xi = v % Di, yi = (v / Di) % Di, zi = (v / (Di * Di)) % Di, b = v / ((size_t)Di * Di * Di);
+    float s = 0.0f;
+    for (int q = 0; q < 8; ++q) {
+      const size_t vo = (((size_t)b * Do + 2 * zi + (q >> 2)) * Do + 2 * yi + ((q >> 1) & 1)) * Do + 2 * xi + (q & 1);
+      s += bf2f(gy[vo * ldy + yofs + c]);
+    }
+    gx[v * ldx + xofs + c] = f2bf(s);
+  }
+}
+
+// input packing: FP32 volume -> BF16 [v][16], channel 0 = value
+__global__ void pack_input_kernel(const float *in, uint16_t *x, size_t nvox)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nvox * 16; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = (i % 16 == 0) ? f2bf(in[i / 16]) : (uint16_t)0;
+}
+
+// 1x1x1 head with residual: out[v] = in[v] + b + sum_c w[c] h[v][c]
+__global__ void head_kernel(const uint16_t *h, const float *in, const float *w, const float *b,
+    float *out, size_t nvox)
+{
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+    float s = in[v] + b[0];
+    const uint16_t *hv = h + v * 16;
+    for (int c = 0; c < 16; ++c)
+      s = fmaf(w[c], bf2f(hv[c]), s);
+    out[v] = s;
+  }
+}
+
+// MSE: loss += sum (out-t)^2 / N;  gy = 2 (out - t) / N
+__global__ void mse_kernel(const float *out, const float *t, float *gy, size_t nvox, float *loss)
+{
+  float s = 0.0f;
+  const float inv = 1.0f / (float)nvox;
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+    const float d = out[v] - t[v];
+    s += d * d;
+    gy[v] = 2.0f * d * inv;
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0)
+    atomicAdd(loss, s * inv);
+}
+
+// head backward: gh[v][c] = gy[v] w[c]; gw[c] += sum gy h; gb += sum gy
+__global__ void head_bwd_kernel(const float *gy, const uint16_t *h, const float *w, uint16_t *gh,
+    float *gw, float *gb, size_t nvox)
+{
+  float sw[16], sb = 0.0f;
+  for (int c = 0; c < 16; ++c)
+    sw[c] = 0.0f;
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+    const float g = gy[v];
+    sb += g;
+    for (int c = 0; c < 16; ++c) {
+      sw[c] = fmaf(g, bf2f(h[v * 16 + c]), sw[c]);
+      gh[v * 16 + c] = f2bf(g * w[c]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sb += __shfl_xor(sb, o);
+    for (int c = 0; c < 16; ++c)
+      sw[c] += __shfl_xor(sw[c], o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(gb, sb);
+    for (int c = 0; c < 16; ++c)
+      atomicAdd(gw + c, sw[c]);
+  }
+}
+
+// Adam on a flat FP32 parameter array (bias-corrected, PyTorch's form)
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, size_t n, float lr,
+    float b1, float b2, float eps, float c1, float c2)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// master W[co][tap][ci] (FP32) -> forward packing wf[co][kpos(cin, tap, ci)]
+// and backward-data packing wb[ci][kpos(cout, 26 - tap, co)] (BF16)
+__global__ void pack_weights_kernel(const float *W, int cin, int cout, uint16_t *wf, int kpf,
+    uint16_t *wb, int kpb)
+{
+  const int total = cout * 27 * cin;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int ci = i % cin, tap = (i / cin) % 27, co = i / (27 * cin);
+    const uint16_t h = f2bf(W[i]);
+    wf[(size_t)co * kpf + kpos(cin, tap, ci)] = h;
+    wb[(size_t)ci * kpb + kpos(cout, 26 - tap, co)] = h;
+  }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+int grid_for(size_t n, int block = 256, int cap = 16384)
+{
+  return (int)std::max<size_t>(1, std::min<size_t>((n + block - 1) / block, cap));
+}
+
+struct Conv {
+  int cin, cout, kpf, kpb;
+  float *W = nullptr, *b = nullptr;     // master FP32
+  float *gW = nullptr, *gb = nullptr;
+  float *mW = nullptr, *vW = nullptr, *mb = nullptr, *vb = nullptr;
+  uint16_t *wf = nullptr, *wb = nullptr;
+};
+
+struct Act {
+  uint16_t *p = nullptr;
+  int ld = 0, ofs = 0;
+};
+
+} // namespace
+
+struct LhUnet {
+  Conv c[10];
+  float *hw = nullptr, *hb = nullptr, *ghw = nullptr, *ghb = nullptr;
+  float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
+  int step = 0;
+  hipStream_t stream = nullptr;
+  // workspace for one (n, D)
+  int wsN = 0, wsD = 0;
+  std::vector<void *> ws;
+  uint16_t *x16, *a1, *cat1, *p1, *a2, *cat2, *p2, *a3, *b3, *d2a, *d2b, *d1a, *d1b;
+  uint16_t *g_a1, *g_cat1, *g_p1, *g_a2, *g_cat2, *g_p2, *g_a3, *g_b3, *g_d2a, *g_d2b, *g_d1a,
+      *g_d1b;
+  float *out, *gy, *loss;
+};
+
+namespace {
+
+// layer table: (cin, cout); order enc1a enc1b enc2a enc2b bot_a bot_b dec2a dec2b dec1a dec1b
+const int kLayers[10][2] = { { 16, 16 }, { 16, 16 }, { 16, 32 }, { 32, 32 }, { 32, 64 },
+  { 64, 64 }, { 96, 32 }, { 32, 32 }, { 48, 16 }, { 16, 16 } };
+
+#define UN_HIP(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return lh_fail(e_ == hipErrorOutOfMemory ? LH_E_NOMEM : LH_E_HIP, "%s:%d %s: %s",    \
+          __FILE__, __LINE__, #call, hipGetErrorString(e_));                             \
+  } while (0)
+
+template <typename T> int dalloc(T **p, size_t n)
+{
+  UN_HIP(hipMalloc((void **)p, n * sizeof(T)));
+  UN_HIP(hipMemset(*p, 0, n * sizeof(T)));
+  return LH_OK;
+}
+
+// rows of 16 voxels per wave for the forward/backward-data kernel
+constexpr int kConvMT = 2;
+
+template <int NT> void launch_conv(hipStream_t s, const ConvArgs &a)
+{
+  const size_t waves = (size_t)a.n * a.D * (a.D / kConvMT) * (a.D / 16);
+  conv3_kernel<NT, kConvMT><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(a);
+}
+
+int conv_fwd(hipStream_t s, const Conv &L, Act in, Act out, int D, int n, bool relu,
+    bool bias, bool accumulate, bool backward)
+{
+  ConvArgs a;
+  a.x = in.p;
+  a.ldx = in.ld;
+  a.xofs = in.ofs;
+  a.y = out.p;
+  a.ldy = out.ld;
+  a.yofs = out.ofs;
+  a.D = D;
+  a.n = n;
+  a.relu = relu;
+  a.accumulate = accumulate;
+  if (!backward) {
+    a.w = L.wf; a.cin = L.cin; a.cout = L.cout; a.kp = L.kpf; a.bias = bias ? L.b : nullptr;
+  } else {
+    a.w = L.wb; a.cin = L.cout; a.cout = L.cin; a.kp = L.kpb; a.bias = nullptr;
+  }
+  if (D % 16 || D < 16)
+    return lh_fail(LH_E_ARG, "conv: D must be a multiple of 16 (got %d)", D);
+  if (a.cin % 16 || a.ldx % 8 || a.xofs % 8)
+    return lh_fail(LH_E_ARG, "conv: input channels/stride must be multiples of 16/8");
+  const uint64_t xBytes = (uint64_t)n * D * D * D * a.ldx * 2;
+  if (xBytes >= kMissOffset)
+    return lh_fail(LH_E_ARG, "conv: input of %llu bytes exceeds the 2 GiB buffer range",
+        (unsigned long long)xBytes);
+  a.xBytes = (int)xBytes;
+  switch (a.cout) {
+  case 16: launch_conv<1>(s, a); break;
+  case 32: launch_conv<2>(s, a); break;
+  case 48: launch_conv<3>(s, a); break;
+  case 64: launch_conv<4>(s, a); break;
+  case 96: launch_conv<6>(s, a); break;
+  default: return lh_fail(LH_E_ARG, "conv: unsupported output channels %d", a.cout);
+  }
+  UN_HIP(hipGetLastError());
+  return LH_OK;
+}
+
+int conv_wgrad(hipStream_t s, const Conv &L, Act x, Act g, int D, int n)
+{
+  WgradArgs a;
+  a.x = x.p; a.ldx = x.ld; a.xofs = x.ofs;
+  a.g = g.p; a.ldg = g.ld; a.gofs = g.ofs;
+  a.dw = L.gW;
+  a.D = D;
+  a.nrows = n * D * D;
+  if (D % 16 || D < 16)
+    return lh_fail(LH_E_ARG, "wgrad: D must be a multiple of 16 (got %d)", D);
+  const int blocksX = std::max(1, std::min(a.nrows, 1024));
+  a.rowsPerBlock = (a.nrows + blocksX - 1) / blocksX;
+  const dim3 grid((a.nrows + a.rowsPerBlock - 1) / a.rowsPerBlock, 9);
+  const int key = L.cout / 16 * 16 + L.cin / 16;
+  switch (key) {
+  case 1 * 16 + 1: wgrad_kernel<1, 1><<<grid, 256, 0, s>>>(a); break;
+  case 2 * 16 + 1: wgrad_kernel<2, 1><<<grid, 256, 0, s>>>(a); break;
+  case 2 * 16 + 2: wgrad_kernel<2, 2><<<grid, 256, 0, s>>>(a); break;
+  case 4 * 16 + 2: wgrad_kernel<4, 2><<<grid, 256, 0, s>>>(a); break;
+  case 4 * 16 + 4: wgrad_kernel<4, 4><<<grid, 256, 0, s>>>(a); break;
+  case 2 * 16 + 6: wgrad_kernel<2, 6><<<grid, 256, 0, s>>>(a); break;
+  case 1 * 16 + 3: wgrad_kernel<1, 3><<<grid, 256, 0, s>>>(a); break;
+  default:
+    return lh_fail(LH_E_ARG, "wgrad: unsupported channels %d -> %d", L.cin, L.cout);
+  }
+  UN_HIP(hipGetLastError());
+  bgrad_kernel<<<256, 256, 0, s>>>(g.p, g.ld, g.ofs, L.cout, n * D * D * D, L.gb);
+  UN_HIP(hipGetLastError());
+  return LH_OK;
+}
+
+int pack_all(LhUnet *u)
+{
+  for (auto &L : u->c) {
+    pack_weights_kernel<<<grid_for((size_t)L.cout * 27 * L.cin), 256, 0, u->stream>>>(L.W, L.cin,
+        L.cout, L.wf, L.kpf, L.wb, L.kpb);
+    UN_HIP(hipGetLastError());
+  }
+  return LH_OK;
+}
+
+int ensure_ws(LhUnet *u, int n, int D)
+{
+  if (u->wsN == n && u->wsD == D)
+    return LH_OK;
+  for (void *p : u->ws)
+    (void)hipFree(p);
+  u->ws.clear();
+  u->wsN = u->wsD = 0;
+  const size_t v1 = (size_t)n * D * D * D, v2 = v1 / 8, v3 = v2 / 8;
+  struct { uint16_t **p; size_t n; } bufs[] = {
+    { &u->x16, v1 * 16 }, { &u->a1, v1 * 16 }, { &u->cat1, v1 * 48 }, { &u->p1, v2 * 16 },
+    { &u->a2, v2 * 32 }, { &u->cat2, v2 * 96 }, { &u->p2, v3 * 32 }, { &u->a3, v3 * 64 },
+    { &u->b3, v3 * 64 }, { &u->d2a, v2 * 32 }, { &u->d2b, v2 * 32 }, { &u->d1a, v1 * 16 },
+    { &u->d1b, v1 * 16 }, { &u->g_a1, v1 * 16 }, { &u->g_cat1, v1 * 48 }, { &u->g_p1, v2 * 16 },
+    { &u->g_a2, v2 * 32 }, { &u->g_cat2, v2 * 96 }, { &u->g_p2, v3 * 32 }, { &u->g_a3, v3 * 64 },
+    { &u->g_b3, v3 * 64 }, { &u->g_d2a, v2 * 32 }, { &u->g_d2b, v2 * 32 }, { &u->g_d1a, v1 * 16 },
+    { &u->g_d1b, v1 * 16 } };
+  for (auto &b : bufs) {
+    int rc = dalloc(b.p, b.n);
+    if (rc != LH_OK)
+      return rc;
+    u->ws.push_back(*b.p);
+  }
+  int rc = dalloc(&u->out, v1);
+  if (rc == LH_OK) { u->ws.push_back(u->out); rc = dalloc(&u->gy, v1); }
+  if (rc == LH_OK) { u->ws.push_back(u->gy); rc = dalloc(&u->loss, 1); }
+  if (rc == LH_OK) u->ws.push_back(u->loss);
+  if (rc != LH_OK)
+    return rc;
+  u->wsN = n;
+  u->wsD = D;
+  return LH_OK;
+}
+
+// forward through the whole net; activations stay in the workspace
+int forward(LhUnet *u, const float *in, int n, int D)
+{
+  if (D % 64)
+    return lh_fail(LH_E_ARG, "U-Net input edge must be a multiple of 64 (got %d)", D);
+  int rc = ensure_ws(u, n, D);
+  if (rc != LH_OK)
+    return rc;
+  hipStream_t s = u->stream;
+  const int D1 = D, D2 = D / 2, D3 = D / 4;
+  const int n1 = n * D1 * D1 * D1, n2 = n * D2 * D2 * D2, n3 = n * D3 * D3 * D3;
+  pack_input_kernel<<<grid_for((size_t)n1 * 16), 256, 0, s>>>(in, u->x16, (size_t)n1);
+  auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
+#define FW(L, i, o, Dd, nv) if ((rc = conv_fwd(s, u->c[L], i, o, Dd, n, true, true, false, false)) != LH_OK) return rc
+  FW(0, A(u->x16, 16), A(u->a1, 16), D1, n1);
+  FW(1, A(u->a1, 16), A(u->cat1, 48, 32), D1, n1);
+  pool_kernel<<<grid_for((size_t)n2 * 16), 256, 0, s>>>(u->cat1, 48, 32, u->p1, 16, 0, 16, D2, n);
+  FW(2, A(u->p1, 16), A(u->a2, 32), D2, n2);
+  FW(3, A(u->a2, 32), A(u->cat2, 96, 64), D2, n2);
+  pool_kernel<<<grid_for((size_t)n3 * 32), 256, 0, s>>>(u->cat2, 96, 64, u->p2, 32, 0, 32, D3, n);
+  FW(4, A(u->p2, 32), A(u->a3, 64), D3, n3);
+  FW(5, A(u->a3, 64), A(u->b3, 64), D3, n3);
+  up_kernel<<<grid_for((size_t)n2 * 64), 256, 0, s>>>(u->b3, 64, 0, u->cat2, 96, 0, 64, D3, n);
+  FW(6, A(u->cat2, 96), A(u->d2a, 32), D2, n2);
+  FW(7, A(u->d2a, 32), A(u->d2b, 32), D2, n2);
+  up_kernel<<<grid_for((size_t)n1 * 32), 256, 0, s>>>(u->d2b, 32, 0, u->cat1, 48, 0, 32, D2, n);
+  FW(8, A(u->cat1, 48), A(u->d1a, 16), D1, n1);
+  FW(9, A(u->d1a, 16), A(u->d1b, 16), D1, n1);
+#undef FW
+  head_kernel<<<grid_for((size_t)n1), 256, 0, s>>>(u->d1b, in, u->hw, u->hb, u->out, (size_t)n1);
+  UN_HIP(hipGetLastError());
+  return LH_OK;
+}
+
+int zero_grads(LhUnet *u)
+{
+  for (auto &L : u->c) {
+    UN_HIP(hipMemsetAsync(L.gW, 0, (size_t)L.cout * 27 * L.cin * sizeof(float), u->stream));
+    UN_HIP(hipMemsetAsync(L.gb, 0, L.cout * sizeof(float), u->stream));
+  }
+  UN_HIP(hipMemsetAsync(u->ghw, 0, 16 * sizeof(float), u->stream));
+  UN_HIP(hipMemsetAsync(u->ghb, 0, sizeof(float), u->stream));
+  return LH_OK;
+}
+
+// backward from gy (already in u->gy) into the parameter gradients
+int backward(LhUnet *u, const float *in, int n, int D)
+{
+  hipStream_t s = u->stream;
+  const int D1 = D, D2 = D / 2, D3 = D / 4;
+  const int n1 = n * D1 * D1 * D1, n2 = n * D2 * D2 * D2, n3 = n * D3 * D3 * D3;
+  const size_t v1 = n1, v2 = n2;
+  int rc = zero_grads(u);
+  if (rc != LH_OK)
+    return rc;
+  auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
+  auto relub = [&](uint16_t *g, const uint16_t *y, int ld, int ofs, int C, int nv) {
+    relu_bwd_kernel<<<grid_for((size_t)nv * C), 256, 0, s>>>(g, y, ld, ofs, C, nv);
+  };
+#define WG(L, x, g, Dd) if ((rc = conv_wgrad(s, u->c[L], x, g, Dd, n)) != LH_OK) return rc
+#define BD(L, g, o, Dd, nv, acc) if ((rc = conv_fwd(s, u->c[L], g, o, Dd, n, false, false, acc, true)) != LH_OK) return rc
+  head_bwd_kernel<<<grid_for(v1), 256, 0, s>>>(u->gy, u->d1b, u->hw, u->g_d1b, u->ghw, u->ghb, v1);
+  relub(u->g_d1b, u->d1b, 16, 0, 16, n1);
+  WG(9, A(u->d1a, 16), A(u->g_d1b, 16), D1);
+  BD(9, A(u->g_d1b, 16), A(u->g_d1a, 16), D1, n1, false);
+  relub(u->g_d1a, u->d1a, 16, 0, 16, n1);
+  WG(8, A(u->cat1, 48), A(u->g_d1a, 16), D1);
+  BD(8, A(u->g_d1a, 16), A(u->g_cat1, 48), D1, n1, false);
+  up_bwd_kernel<<<grid_for(v2 * 32), 256, 0, s>>>(u->g_cat1, 48, 0, u->g_d2b, 32, 0, 32, D2, n);
+  relub(u->g_d2b, u->d2b, 32, 0, 32, n2);
+  WG(7, A(u->d2a, 32), A(u->g_d2b, 32), D2);
+  BD(7, A(u->g_d2b, 32), A(u->g_d2a, 32), D2, n2, false);
+  relub(u->g_d2a, u->d2a, 32, 0, 32, n2);
+  WG(6, A(u->cat2, 96), A(u->g_d2a, 32), D2);
+  BD(6, A(u->g_d2a, 32), A(u->g_cat2, 96), D2, n2, false);
+  up_bwd_kernel<<<grid_for((size_t)n3 * 64), 256, 0, s>>>(u->g_cat2, 96, 0, u->g_b3, 64, 0, 64, D3, n);
+  relub(u->g_b3, u->b3, 64, 0, 64, n3);
+  WG(5, A(u->a3, 64), A(u->g_b3, 64), D3);
+  BD(5, A(u->g_b3, 64), A(u->g_a3, 64), D3, n3, false);
+  relub(u->g_a3, u->a3, 64, 0, 64, n3);
+  WG(4, A(u->p2, 32), A(u->g_a3, 64), D3);
+  BD(4, A(u->g_a3, 64), A(u->g_p2, 32), D3, n3, false);
+  pool_bwd_kernel<<<grid_for(v2 * 32), 256, 0, s>>>(u->g_p2, 32, 0, u->g_cat2, 96, 64, 32, D2, n);
+  relub(u->g_cat2, u->cat2, 96, 64, 32, n2);
+  WG(3, A(u->a2, 32), A(u->g_cat2, 96, 64), D2);
+  BD(3, A(u->g_cat2, 96, 64), A(u->g_a2, 32), D2, n2, false);
+  relub(u->g_a2, u->a2, 32, 0, 32, n2);
+  WG(2, A(u->p1, 16), A(u->g_a2, 32), D2);
+  BD(2, A(u->g_a2, 32), A(u->g_p1, 16), D2, n2, false);
+  pool_bwd_kernel<<<grid_for(v1 * 16), 256, 0, s>>>(u->g_p1, 16, 0, u->g_cat1, 48, 32, 16, D1, n);
+  relub(u->g_cat1, u->cat1, 48, 32, 16, n1);
+  WG(1, A(u->a1, 16), A(u->g_cat1, 48, 32), D1);
+  BD(1, A(u->g_cat1, 48, 32), A(u->g_a1, 16), D1, n1, false);
+  relub(u->g_a1, u->a1, 16, 0, 16, n1);
+  WG(0, A(u->x16, 16), A(u->g_a1, 16), D1);
+#undef WG
+#undef BD
+  (void)in;
+  UN_HIP(hipGetLastError());
+  return LH_OK;
+}
+
+int adam_all(LhUnet *u, float lr)
+{
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  u->step += 1;
+  const float c1 = 1.0f - powf(b1, (float)u->step), c2 = 1.0f - powf(b2, (float)u->step);
+  hipStream_t s = u->stream;
+  for (auto &L : u->c) {
+    const size_t nw = (size_t)L.cout * 27 * L.cin;
+    adam_kernel<<<grid_for(nw), 256, 0, s>>>(L.W, L.gW, L.mW, L.vW, nw, lr, b1, b2, eps, c1, c2);
+    adam_kernel<<<1, 256, 0, s>>>(L.b, L.gb, L.mb, L.vb, L.cout, lr, b1, b2, eps, c1, c2);
+  }
+  adam_kernel<<<1, 256, 0, s>>>(u->hw, u->ghw, u->mhw, u->vhw, 16, lr, b1, b2, eps, c1, c2);
+  adam_kernel<<<1, 256, 0, s>>>(u->hb, u->ghb, u->mhb, u->vhb, 1, lr, b1, b2, eps, c1, c2);
+  UN_HIP(hipGetLastError());
+  return pack_all(u);
+}
+
+} // namespace
+
+extern "C" {
+
+int lhUnetCreate(LhUnet **out, uint64_t seed)
+{
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return lh_fail(LH_E_NODEV, "no HIP device visible");
+  LhUnet *u = new LhUnet;
+  *out = u;
+  UN_HIP(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking));
+  std::mt19937_64 rng(seed);
+  for (int i = 0; i < 10; ++i) {
+    Conv &L = u->c[i];
+    L.cin = kLayers[i][0];
+    L.cout = kLayers[i][1];
+    L.kpf = 32 * kchunks(L.cin);
+    L.kpb = 32 * kchunks(L.cout);
+    const size_t nw = (size_t)L.cout * 27 * L.cin;
+    int rc;
+    if ((rc = dalloc(&L.W, nw)) || (rc = dalloc(&L.b, L.cout)) || (rc = dalloc(&L.gW, nw)) ||
+        (rc = dalloc(&L.gb, L.cout)) || (rc = dalloc(&L.mW, nw)) || (rc = dalloc(&L.vW, nw)) ||
+        (rc = dalloc(&L.mb, L.cout)) || (rc = dalloc(&L.vb, L.cout)) ||
+        (rc = dalloc(&L.wf, (size_t)L.cout * L.kpf)) || (rc = dalloc(&L.wb, (size_t)L.cin * L.kpb)))
+      return rc;
+    // He-uniform initialisation; the first layer sees one live input channel
+    const int fanIn = 27 * (i == 0 ? 1 : L.cin);
+    const float bound = sqrtf(6.0f / (float)fanIn);
+    std::uniform_real_distribution<float> dist(-bound, bound);
+    std::vector<float> h(nw);
+    for (size_t k = 0; k < nw; ++k)
+      h[k] = (i == 0 && (k % L.cin) != 0) ? 0.0f : dist(rng);
+    UN_HIP(hipMemcpy(L.W, h.data(), nw * sizeof(float), hipMemcpyHostToDevice));
+  }
+  int rc;
+  if ((rc = dalloc(&u->hw, 16)) || (rc = dalloc(&u->hb, 1)) || (rc = dalloc(&u->ghw, 16)) ||
+      (rc = dalloc(&u->ghb, 1)) || (rc = dalloc(&u->mhw, 16)) || (rc = dalloc(&u->vhw, 16)) ||
+      (rc = dalloc(&u->mhb, 1)) || (rc = dalloc(&u->vhb, 1)))
+    return rc;
+  // the head starts near zero so the untrained net is close to the identity
+  std::vector<float> hw(16);
+  std::uniform_real_distribution<float> hd(-1e-3f, 1e-3f);
+  for (auto &w : hw)
+    w = hd(rng);
+  UN_HIP(hipMemcpy(u->hw, hw.data(), 16 * sizeof(float), hipMemcpyHostToDevice));
+  if ((rc = pack_all(u)) != LH_OK)
+    return rc;
+  UN_HIP(hipStreamSynchronize(u->stream));
+  return LH_OK;
+}
+
+int lhUnetDestroy(LhUnet *u)
+{
+  if (!u)
+    return LH_OK;
+  if (u->stream)
+    (void)hipStreamSynchronize(u->stream);
+  for (auto &L : u->c)
+    for (void *p : { (void *)L.W, (void *)L.b, (void *)L.gW, (void *)L.gb, (void *)L.mW, (void *)L.vW,
+             (void *)L.mb, (void *)L.vb, (void *)L.wf, (void *)L.wb })
+      (void)hipFree(p);
+  for (void *p : { (void *)u->hw, (void *)u->hb, (void *)u->ghw, (void *)u->ghb, (void *)u->mhw,
+           (void *)u->vhw, (void *)u->mhb, (void *)u->vhb })
+    (void)hipFree(p);
+  for (void *p : u->ws)
+    (void)hipFree(p);
+  if (u->stream)
+    (void)hipStreamDestroy(u->stream);
+  delete u;
+  return LH_OK;
+}
+
+uint64_t lhUnetParamCount(void)
+{
+  uint64_t s = 17;
+  for (auto &l : kLayers)
+    s += (uint64_t)l[1] * 27 * l[0] + l[1];
+  return s;
+}
+
+// Parameters in a flat FP32 host array, layer by layer: W[co][tap][ci] then b,
+// then the head weights (16) and bias (1).
+int lhUnetGetParams(LhUnet *u, float *host)
+{
+  if (!u || !host)
+    return lh_fail(LH_E_ARG, "lhUnetGetParams: null argument");
+  UN_HIP(hipStreamSynchronize(u->stream));
+  size_t o = 0;
+  for (auto &L : u->c) {
+    const size_t nw = (size_t)L.cout * 27 * L.cin;
+    UN_HIP(hipMemcpy(host + o, L.W, nw * sizeof(float), hipMemcpyDeviceToHost));
+    o += nw;
+    UN_HIP(hipMemcpy(host + o, L.b, L.cout * sizeof(float), hipMemcpyDeviceToHost));
+    o += L.cout;
+  }
+  UN_HIP(hipMemcpy(host + o, u->hw, 16 * sizeof(float), hipMemcpyDeviceToHost));
+  UN_HIP(hipMemcpy(host + o + 16, u->hb, sizeof(float), hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+int lhUnetSetParams(LhUnet *u, const float *host)
+{
+  if (!u || !host)
+    return lh_fail(LH_E_ARG, "lhUnetSetParams: null argument");
+  UN_HIP(hipStreamSynchronize(u->stream));
+  size_t o = 0;
+  for (auto &L : u->c) {
+    const size_t nw = (size_t)L.cout * 27 * L.cin;
+    UN_HIP(hipMemcpy(L.W, host + o, nw * sizeof(float), hipMemcpyHostToDevice));
+    o += nw;
+    UN_HIP(hipMemcpy(L.b, host + o, L.cout * sizeof(float), hipMemcpyHostToDevice));
+    o += L.cout;
+  }
+  UN_HIP(hipMemcpy(u->hw, host + o, 16 * sizeof(float), hipMemcpyHostToDevice));
+  UN_HIP(hipMemcpy(u->hb, host + o + 16, sizeof(float), hipMemcpyHostToDevice));
+  int rc = pack_all(u);
+  if (rc != LH_OK)
+    return rc;
+  UN_HIP(hipStreamSynchronize(u->stream));
+  return LH_OK;
+}
+
+// Gradients of the last backward pass, same flat layout as the parameters.
+int lhUnetGetGrads(LhUnet *u, float *host)
+{
+  if (!u || !host)
+    return lh_fail(LH_E_ARG, "lhUnetGetGrads: null argument");
+  UN_HIP(hipStreamSynchronize(u->stream));
+  size_t o = 0;
+  for (auto &L : u->c) {
+    const size_t nw = (size_t)L.cout * 27 * L.cin;
+    UN_HIP(hipMemcpy(host + o, L.gW, nw * sizeof(float), hipMemcpyDeviceToHost));
+    o += nw;
+    UN_HIP(hipMemcpy(host + o, L.gb, L.cout * sizeof(float), hipMemcpyDeviceToHost));
+    o += L.cout;
+  }
+  UN_HIP(hipMemcpy(host + o, u->ghw, 16 * sizeof(float), hipMemcpyDeviceToHost));
+  UN_HIP(hipMemcpy(host + o + 16, u->ghb, sizeof(float), hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+// Inference: dOut (device, n*D^3 floats) = net(dIn); D a multiple of 64.
+int lhUnetForward(LhUnet *u, const float *dIn, float *dOut, int n, int D)
+{
+  if (!u || !dIn || !dOut || n < 1)
+    return lh_fail(LH_E_ARG, "lhUnetForward: bad arguments");
+  int rc = forward(u, dIn, n, D);
+  if (rc != LH_OK)
+    return rc;
+  const size_t nv = (size_t)n * D * D * D;
+  UN_HIP(hipMemcpyAsync(dOut, u->out, nv * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
+  UN_HIP(hipStreamSynchronize(u->stream));
+  return LH_OK;
+}
+
+// Loss and gradients for inputs dIn, targets dTarget (device, n*D^3 floats);
+// with lr > 0 also one Adam step.  *loss = mean squared error.
+int lhUnetTrainStep(LhUnet *u, const float *dIn, const float *dTarget, int n, int D, float lr,
+    float *loss)
+{
+  if (!u || !dIn || !dTarget || n < 1)
+    return lh_fail(LH_E_ARG, "lhUnetTrainStep: bad arguments");
+  int rc = forward(u, dIn, n, D);
+  if (rc != LH_OK)
+    return rc;
+  const size_t nv = (size_t)n * D * D * D;
+  UN_HIP(hipMemsetAsync(u->loss, 0, sizeof(float), u->stream));
+  mse_kernel<<<grid_for(nv), 256, 0, u->stream>>>(u->out, dTarget, u->gy, nv, u->loss);
+  UN_HIP(hipGetLastError());
+  if ((rc = backward(u, dIn, n, D)) != LH_OK)
+    return rc;
+  if (lr > 0.0f && (rc = adam_all(u, lr)) != LH_OK)
+    return rc;
+  float l = 0.0f;
+  UN_HIP(hipMemcpyAsync(&l, u->loss, sizeof(float), hipMemcpyDeviceToHost, u->stream));
+  UN_HIP(hipStreamSynchronize(u->stream));
+  if (loss)
+    *loss = l;
+  return LH_OK;
+}
+
+// Single convolution entry for tests: x [n*D^3][cin] bf16 (device),
+// W [cout][27][cin] FP32 host, bias FP32 host or null, y [n*D^3][cout] bf16.
+// backward = 1 runs the backward-data pass (x holds cout channels, y cin).
+int lhConv3dTest(const uint16_t *dx, int cin, int cout, const float *W, const float *b,
+    uint16_t *dy, int n, int D, int relu, int backward)
+{
+  Conv L;
+  L.cin = cin;
+  L.cout = cout;
+  L.kpf = 32 * kchunks(cin);
+  L.kpb = 32 * kchunks(cout);
+  const size_t nw = (size_t)cout * 27 * cin;
+  int rc;
+  if ((rc = dalloc(&L.W, nw)) || (rc = dalloc(&L.b, cout)) || (rc = dalloc(&L.wf, (size_t)cout * L.kpf)) ||
+      (rc = dalloc(&L.wb, (size_t)cin * L.kpb)))
+    return rc;
+  UN_HIP(hipMemcpy(L.W, W, nw * sizeof(float), hipMemcpyHostToDevice));
+  if (b)
+    UN_HIP(hipMemcpy(L.b, b, cout * sizeof(float), hipMemcpyHostToDevice));
+  pack_weights_kernel<<<grid_for(nw), 256>>>(L.W, cin, cout, L.wf, L.kpf, L.wb, L.kpb);
+  Act in, out;
+  in.p = (uint16_t *)dx; in.ld = backward ? cout : cin; in.ofs = 0;
+  out.p = dy; out.ld = backward ? cin : cout; out.ofs = 0;
+  rc = conv_fwd(0, L, in, out, D, n, relu && !backward, b != nullptr && !backward,
+      false, backward != 0);
+  UN_HIP(hipDeviceSynchronize());
+  for (void *p : { (void *)L.W, (void *)L.b, (void *)L.wf, (void *)L.wb })
+    (void)hipFree(p);
+  return rc;
+}
+
+// Weight-gradient entry for tests: dW [cout][27][cin] FP32 host output.
+int lhConv3dWgradTest(const uint16_t *dx, int cin, const uint16_t *dg, int cout, float *dW,
+    int n, int D)
+{
+  Conv L;
+  L.cin = cin;
+  L.cout = cout;
+  const size_t nw = (size_t)cout * 27 * cin;
+  int rc;
+  if ((rc = dalloc(&L.gW, nw)) || (rc = dalloc(&L.gb, cout)))
+    return rc;
+  Act x, g;
+  x.p = (uint16_t *)dx; x.ld = cin; x.ofs = 0;
+  g.p = (uint16_t *)dg; g.ld = cout; g.ofs = 0;
+  rc = conv_wgrad(0, L, x, g, D, n);
+  UN_HIP(hipDeviceSynchronize());
+  UN_HIP(hipMemcpy(dW, L.gW, nw * sizeof(float), hipMemcpyDeviceToHost));
+  (void)hipFree(L.gW);
+  (void)hipFree(L.gb);
+  return rc;
+}
+
+// Device memory helpers for callers without another allocator.
+int lhDeviceAlloc(void **p, uint64_t bytes)
+{
+  UN_HIP(hipMalloc(p, bytes));
+  return LH_OK;
+}
+int lhDeviceFree(void *p)
+{
+  UN_HIP(hipFree(p));
+  return LH_OK;
+}
+int lhCopyToDevice(void *dst, const void *src, uint64_t bytes)
+{
+  UN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return LH_OK;
+}
+int lhCopyToHost(void *dst, const void *src, uint64_t bytes)
+{
+  UN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return LH_OK;
+}
+
+} // extern "C"

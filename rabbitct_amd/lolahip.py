@@ -73,7 +73,12 @@ EXPORTS = (
     "lhSetRangesFromGeometry", "lhSetVolume", "lhZeroVolume", "lhGetVolume",
     "lhLoadView", "lhLoadViewDevice", "lhBackproject", "lhTimedBackproject",
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
-    "lhSetLaunch", "lhCopyCalibration",
+    "lhSetLaunch", "lhCopyCalibration", "lhVolumeDevice", "lhScaleVolume",
+    # sparse-view + learned correction
+    "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
+    "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
+    "lhUnetGetGrads", "lhUnetForward", "lhUnetTrainStep", "lhConv3dTest", "lhConv3dWgradTest",
+    "lhDeviceAlloc", "lhDeviceFree", "lhCopyToDevice", "lhCopyToHost",
 )
 
 _lib = None
@@ -125,6 +130,28 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhCopyCalibration.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(f32)]
     lib.lhAbiLayout.argtypes = [P]
     lib.lhAbiLayout.restype = None
+    lib.lhVolumeDevice.argtypes = [P, ctypes.POINTER(P)]
+    lib.lhScaleVolume.argtypes = [P, f32]
+    u64, f64 = ctypes.c_uint64, ctypes.c_double
+    lib.lhLineIntegrals.argtypes = [i32, i32, i32, u32, P, P, u32, P, P]
+    lib.lhForwardProject.argtypes = [P, i32, f32, f32, i32, i32, i32, u32, P, P, f32, P]
+    lib.lhRelativeResidual.argtypes = [P, P, u64, ctypes.POINTER(f64)]
+    lib.lhExtractPatches.argtypes = [P, i32, i32, P, i32, i32, P]
+    lib.lhUnetCreate.argtypes = [ctypes.POINTER(P), u64]
+    lib.lhUnetDestroy.argtypes = [P]
+    lib.lhUnetParamCount.argtypes = []
+    lib.lhUnetParamCount.restype = u64
+    lib.lhUnetGetParams.argtypes = [P, P]
+    lib.lhUnetSetParams.argtypes = [P, P]
+    lib.lhUnetGetGrads.argtypes = [P, P]
+    lib.lhUnetForward.argtypes = [P, P, P, i32, i32]
+    lib.lhUnetTrainStep.argtypes = [P, P, P, i32, i32, f32, ctypes.POINTER(f32)]
+    lib.lhConv3dTest.argtypes = [P, i32, i32, P, P, P, i32, i32, i32, i32]
+    lib.lhConv3dWgradTest.argtypes = [P, i32, P, i32, P, i32, i32]
+    lib.lhDeviceAlloc.argtypes = [ctypes.POINTER(P), u64]
+    lib.lhDeviceFree.argtypes = [P]
+    lib.lhCopyToDevice.argtypes = [P, P, u64]
+    lib.lhCopyToHost.argtypes = [P, P, u64]
     _lib = lib
     return lib
 
@@ -226,6 +253,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def scale_volume(self, s: float) -> None:
+        _check(self.lib.lhScaleVolume(self._h, s), "lhScaleVolume")
+
+    def volume_device(self) -> int:
+        """Device address of the resident volume (borrowed)."""
+        p = ctypes.c_void_p()
+        _check(self.lib.lhVolumeDevice(self._h, ctypes.byref(p)), "lhVolumeDevice")
+        return p.value
 
     def set_ranges(self, ranges):
         r = np.ascontiguousarray(ranges, dtype=np.int16)

@@ -21,7 +21,7 @@ L = 128
 def test_library_exports_every_declared_symbol():
     lib = lolahip.load_library()
     header = open(os.path.join(ROOT, "include", "lolahip.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|void|uint64_t|const char \*)\s*\**\s*(l[a-zA-Z]+)\(",
+    declared = set(re.findall(r"^\s*(?:int|void|uint64_t|const char \*)\s*\**\s*(l[a-zA-Z0-9]+)\(",
                               header, re.M))
     assert declared == set(lolahip.EXPORTS)
     for name in declared:
