@@ -16,6 +16,13 @@ phantom, generated on the GPU).  Reported:
                launch stream), against the 8 TB/s HBM peak
   cpu_baseline the reference's own LolaASM (and LolaOPT) from oracle/_ref,
                run by the reference runner on this host on a sample of views
+  sparse_cnn   the second half of the metric, at L=256 (R_L=1 mm): volumes/s
+               of the sparse-view + U-Net pipeline (124 of 496 views, fused
+               FP32 backprojection, U-Net, held-out-view residual check with
+               the full-view fallback) against full-view plain FP32
+               backprojection of all 496 views, on a scan none of the training
+               pairs came from, with the RMSE of both sparse results against
+               the full-view volume (fraction of its range)
 """
 from __future__ import annotations
 
@@ -40,6 +47,7 @@ from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry  # noqa: E402
 METRIC = "GUPS at L=512 plain vs fused/bf16; volumes/s sparse-view+CNN at fixed RMSE"
 HBM_PEAK = 8.0e12
 SEED = 512
+EVAL_SCAN = 200000   # sparse-view scan index (training uses 0..249, validation 100000..)
 
 
 def log(msg):
@@ -120,6 +128,54 @@ def cpu_baseline(args, geom, phantom, ranges):
                               f"{out['LolaOPT']['seconds']:.3f} s"}
 
 
+def sparse_cnn(args):
+    """Volumes/s of full-view plain vs sparse-view + CNN at L=256."""
+    from rabbitct_amd import learned
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        log(f"sparse_cnn: no trained weights at {learned.DEFAULT_WEIGHTS}")
+        return None
+    L = 256
+    t0 = time.time()
+    scan = learned.make_scan(L, EVAL_SCAN)
+    rec = learned.SparseViewReconstructor()
+    log(f"sparse_cnn: scan {EVAL_SCAN} at L={L} generated in {time.time() - t0:.1f} s, "
+        f"threshold {rec.threshold:.4g}")
+
+    def timed(f, n):
+        for _ in range(args.warmup):
+            f()
+        t = time.perf_counter()
+        for _ in range(n):
+            r = f()
+        return (time.perf_counter() - t) / n * 1e3, r
+
+    ms_full, _ = timed(lambda: scan.full_view(*rec.full_kernel), args.steps)
+    full = scan.ctx.get_volume().astype(np.float64)
+    ms_pipe, res = timed(lambda: rec.reconstruct(scan), args.steps)
+    vol = rec.download(res, L)
+    scan.sparse_view(*rec.sparse_kernel)
+    sparse = scan.ctx.get_volume()
+    out = {
+        "workload": f"L={L} (R_L=1 mm), 496-view scan {EVAL_SCAN} (seeded, not in training), "
+                    f"sparse = every 4th view ({scan.n_sparse}) x4, fused FP32; full = all 496 "
+                    "views, plain FP32",
+        "volumes_per_s_full_plain": round(1e3 / ms_full, 3),
+        "volumes_per_s_sparse_cnn": round(1e3 / ms_pipe, 3),
+        "speedup": round(ms_full / ms_pipe, 3),
+        "ms_full_plain": round(ms_full, 3),
+        "ms_sparse_cnn": round(ms_pipe, 3),
+        "ms_stages": {k: round(v, 3) for k, v in res["ms"].items()},
+        "rmse_sparse_cnn_vs_full": learned.rmse_of_range(vol, full),
+        "rmse_sparse_only_vs_full": learned.rmse_of_range(sparse, full),
+        "fallback": res["fallback"], "residual": res["residual"], "threshold": rec.threshold,
+    }
+    log(f"sparse_cnn: full plain {ms_full:.2f} ms, sparse+CNN {ms_pipe:.2f} ms "
+        f"({out['speedup']}x), rmse {out['rmse_sparse_cnn_vs_full']:.4g} "
+        f"(sparse only {out['rmse_sparse_only_vs_full']:.4g}), fallback {res['fallback']}")
+    scan.close()
+    return out
+
+
 def measured_traffic(variant, shape, L):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), already scaled by the
@@ -152,6 +208,8 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a device-to-device copy of this size (HBM counter scale)")
     ap.add_argument("--tune-views", type=int, default=128)
+    ap.add_argument("--no-sparse-cnn", action="store_true",
+                    help="skip the sparse-view + CNN volumes/s measurement at L=256")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -236,6 +294,7 @@ def main():
                      "bytes_per_update": bytes_per_update,
                      "updates_per_launch": int(eff / launches)},
         "cpu_baseline": None,
+        "sparse_cnn": None if args.no_sparse_cnn else sparse_cnn(args),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, geom, phantom, ranges)

@@ -198,7 +198,8 @@ int lhCopyCalibration(LhContext *ctx, uint64_t bytes, float *ms);
 uint64_t lhEffectiveUpdates(LhContext *ctx, uint32_t count);
 
 /* Synthetic input (no counterpart in the reference): for views
- * [firstId, firstId+count) fills slots [firstSlot, ...) with ramp-filtered,
+ * ids[0..count) (1-based; they select the line ranges) fills slots
+ * [firstSlot, firstSlot+count) with ramp-filtered,
  * cosine- and redundancy-weighted analytic line integrals of an ellipsoid
  * phantom.  Per view v: src[3v..] source position, minv[9v..] row-major
  * inverse of the matrix's 3x3 part (ray through pixel (i,j) along
@@ -207,7 +208,7 @@ uint64_t lhEffectiveUpdates(LhContext *ctx, uint32_t count);
  * 9 row-major); tau = ramp-filter sample spacing (mm), dbeta = angular step
  * (rad), border = zero the outermost pixel ring.  hostImages (optional,
  * count*H*W floats) receives the float32 projections. */
-int lhGenerateViews(LhContext *ctx, uint32_t firstSlot, uint32_t count, int firstId,
+int lhGenerateViews(LhContext *ctx, uint32_t firstSlot, uint32_t count, const int *ids,
     const double *matrices, const double *src, const double *minv, const double *axis,
     const float *parker, uint32_t nEll, const double *ell, double tau, double dbeta,
     int border, float *hostImages);

@@ -162,16 +162,20 @@ extern "C" int lhLineIntegrals(int W, int H, int stride, uint32_t count, const d
   return rc;
 }
 
-extern "C" int lhGenerateViews(LhContext *c, uint32_t firstSlot, uint32_t count, int firstId,
+extern "C" int lhGenerateViews(LhContext *c, uint32_t firstSlot, uint32_t count, const int *ids,
     const double *matrices, const double *src, const double *minv, const double *axis,
     const float *parker, uint32_t nEll, const double *ell, double tau, double dbeta,
     int border, float *hostImages)
 {
-  if (!c || !matrices || !src || !minv || !axis || !parker || !ell || nEll == 0 || nEll > 64)
+  if (!c || !ids || !matrices || !src || !minv || !axis || !parker || !ell || nEll == 0 ||
+      nEll > 64)
     return lh_fail(LH_E_ARG, "lhGenerateViews: bad arguments");
   const LhConfig &g = lh_config(c);
-  if (firstSlot + count > g.nSlots || firstId < 1 || (uint32_t)firstId + count - 1 > g.nViews)
-    return lh_fail(LH_E_ARG, "lhGenerateViews: slots/ids out of range");
+  if (firstSlot + count > g.nSlots)
+    return lh_fail(LH_E_ARG, "lhGenerateViews: slots out of range");
+  for (uint32_t v = 0; v < count; ++v)
+    if (ids[v] < 1 || (uint32_t)ids[v] > g.nViews)
+      return lh_fail(LH_E_ARG, "lhGenerateViews: view id %d out of range", ids[v]);
   const int W = (int)g.W, H = (int)g.H;
   hipStream_t s = lh_stream(c);
   double *dGeo = nullptr, *dEll = nullptr, *dInt = nullptr;
@@ -210,7 +214,7 @@ extern "C" int lhGenerateViews(LhContext *c, uint32_t firstSlot, uint32_t count,
         dPar + (size_t)v * W, (int)nEll, dEll, dInt);
     ramp_kernel<<<H, 256, W * sizeof(double), s>>>(W, H, tau, dbeta, border, dInt, dImg);
     LH_TRY(hipGetLastError());
-    rc = lhLoadViewDevice(c, firstSlot + v, firstId + (int)v, matrices + 12 * v, dImg);
+    rc = lhLoadViewDevice(c, firstSlot + v, ids[v], matrices + 12 * v, dImg);
     if (rc != LH_OK)
       goto done;
     if (hostImages)

@@ -63,124 +63,164 @@ __host__ __device__ __forceinline__ int kpos(int C, int tap, int ci)
   return (27 * F + tap / 2) * 32 + (tap & 1) * 16 + (ci - 32 * F);
 }
 
-struct ConvArgs {
-  const uint16_t *x;  // input [vox][ldx], channels [xofs, xofs+cin)
-  const uint16_t *w;  // packed weights [cout][kp] (bf16)
-  const float *bias;  // [cout] or null
-  uint16_t *y;        // output [vox][ldy], channels [yofs, yofs+cout)
-  int ldx, xofs, ldy, yofs;
-  int D, n;           // cube edge, samples
-  int cin, cout, kp;
-  int relu, accumulate;
-  int xBytes;         // bytes addressable from x (buffer range, < 2^31)
+// ---------------------------------------------------------------------------
+// LDS-staged form.  A block computes a 16 (x) x 4 (y) x MT (z) tile of output
+// voxels; wave w owns the MT x-rows at y = w.  The input halo
+// 18 x 6 x (MT+2) voxels x CC channels is staged in LDS once per pass of CC
+// input channels (16-byte loads, zeros outside the volume), and every A
+// operand is one ds_read_b128 of 16 consecutive voxels x 8 channels.  A
+// weight operand (global, L1/L2-resident) feeds MT MFMAs.
+//
+// Input channels [0, c0) may come from a tensor at half resolution (the
+// nearest-neighbour upsampling of the decoder, read in place), channels
+// [c0, cin) from a full-resolution tensor: the concatenation is never
+// materialised.
+struct ConvSrc {
+  const uint16_t *p;  // tensor base
+  int ld, ofs;        // voxel stride and channel offset (elements)
+  int half;           // 1: the tensor has edge D/2 and is read at (x/2, y/2, z/2)
+  int bytes;          // addressable bytes from p (buffer range, < 2^31)
 };
 
-__device__ __forceinline__ int tap_offset(int tap, int D)
+struct ConvLdsArgs {
+  ConvSrc s0, s1;     // channels [0, c0) from s0, [c0, cin) from s1
+  int c0;
+  const uint16_t *w;  // packed weights [cout][kp]
+  const float *bias;
+  uint16_t *y;
+  int ldy, yofs;
+  int D, n, cin, kp;
+  int relu, accumulate;
+  int tilesPerXcd;    // > 0: tiles split into 8 contiguous ranges, one per XCD
+};
+
+constexpr int kHX = 18, kHY = 6;
+
+template <int CC, int MT>
+__device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, int pass, int n,
+    int x0, int y0, int z0)
 {
-  return ((tap / 9 - 1) * D + ((tap / 3) % 3 - 1)) * D + (tap % 3 - 1);
+  constexpr int Q = CC / 8;                // 16-byte units per voxel
+  constexpr int HV = kHX * kHY * (MT + 2); // halo voxels
+  const int D = a.D;
+  for (int i = threadIdx.x; i < HV * Q; i += 256) {
+    const int q = i % Q, v = i / Q;
+    const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+    const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+                    (unsigned)gz < (unsigned)D;
+    const int c = pass * CC + 8 * q;
+    const bool first = c < a.c0;
+    const ConvSrc &sr = first ? a.s0 : a.s1;
+    const int cc = first ? c : c - a.c0;
+    const int Ds = sr.half ? (D >> 1) : D;
+    const int sx = sr.half ? (gx >> 1) : gx, sy = sr.half ? (gy >> 1) : gy,
+              sz = sr.half ? (gz >> 1) : gz;
+    const uint32_t off = in ? (uint32_t)(((((uint32_t)n * Ds + sz) * Ds + sy) * Ds + sx) *
+                                         (uint32_t)sr.ld + sr.ofs + cc) * 2u
+                            : kMissOffset;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+    const u32x4 val = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    *reinterpret_cast<u32x4 *>(lds + v * CC + 8 * q) = val;
+  }
 }
 
-// One wave: MT rows of 16 output voxels (x runs, consecutive y) x NT*16
-// output channels.  Each B (weight) operand feeds MT MFMAs, each A operand NT.
-template <int NT, int MT>
-__global__ __launch_bounds__(256) void conv3_kernel(ConvArgs a)
+template <int CC, int NT, int MT>
+__global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
 {
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int D    = a.D;
-  const int xbN  = D / 16, ybN = D / MT;
-  if (wave >= a.n * D * ybN * xbN)
-    return;
-  const int xb = wave % xbN;
-  int rest     = wave / xbN;
-  const int yb = rest % ybN;
-  rest /= ybN;
-  const int z = rest % D, n = rest / D;
+  extern __shared__ __align__(16) uint16_t halo[];
+  const int D = a.D;
+  const int txN = D / 16, tyN = D / 4, tzN = D / MT;
+  int tile = blockIdx.x;
+  if (a.tilesPerXcd > 0)
+    tile = (blockIdx.x & 7) * a.tilesPerXcd + (blockIdx.x >> 3);
+  const int tx = tile % txN;
+  int rest = tile / txN;
+  const int ty = rest % tyN;
+  rest /= tyN;
+  const int tz = rest % tzN, n = rest / tzN;
+  const int x0 = tx * 16, y0 = ty * 4, z0 = tz * MT;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int m = lane & 15, kg = lane >> 4;
-  const int x = xb * 16 + m, y0 = yb * MT;
-  // in-bounds masks of the 27 taps for this lane's voxel in each row
-  uint32_t mask[MT];
-#pragma unroll
-  for (int r = 0; r < MT; ++r)
-    mask[r] = 0;
-  for (int t = 0; t < 27; ++t) {
-    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-    const bool okzx = (unsigned)(z + dz) < (unsigned)D && (unsigned)(x + dx) < (unsigned)D;
-#pragma unroll
-    for (int r = 0; r < MT; ++r) {
-      const bool ok = okzx && (unsigned)(y0 + r + dy) < (unsigned)D;
-      mask[r] |= (ok ? 1u : 0u) << t;
-    }
-  }
-  const size_t v0 = (((size_t)n * D + z) * D + y0) * D + x;
-  const uint32_t rowStride = (uint32_t)D * a.ldx * 2u;     // bytes between y rows
-  const uint32_t xoff = (uint32_t)((v0 * a.ldx + a.xofs + 8 * kg) * 2u);
-  const __amdgpu_buffer_rsrc_t xr =
-      __builtin_amdgcn_make_buffer_rsrc((void *)a.x, (short)0, a.xBytes, kRsrcWord3);
   const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
   const size_t wtile = (size_t)16 * a.kp;
+  const int F = a.cin / 32;                  // 32-channel blocks in section 1
   f32x4 acc[MT][NT];
 #pragma unroll
   for (int r = 0; r < MT; ++r)
 #pragma unroll
     for (int t = 0; t < NT; ++t)
       acc[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const int F = a.cin / 32;
-  // section 1: (tap, 32-channel block); out-of-volume taps read zeros
-  int chunk = 0;
-  for (int tap = 0; tap < 27; ++tap) {
-    const uint32_t off = xoff + (uint32_t)(tap_offset(tap, D) * a.ldx * 2);
-    for (int b = 0; b < F; ++b, ++chunk) {
-      bf16x8 A[MT];
+  const int passes = a.cin / CC;
+  for (int pass = 0; pass < passes; ++pass) {
+    if (pass)
+      __syncthreads();
+    stage_halo<CC, MT>(a, halo, pass, n, x0, y0, z0);
+    __syncthreads();
+    // section 1: 32-channel blocks of this pass, one tap per chunk
+    constexpr int B32 = CC / 32;
 #pragma unroll
-      for (int r = 0; r < MT; ++r) {
-        const uint32_t o = ((mask[r] >> tap) & 1u) ? off + r * rowStride + 64u * b : kMissOffset;
-        A[r] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(xr, o, 0, 0));
+    for (int b = 0; b < B32; ++b) {
+      const int blk = pass * B32 + b;        // global 32-channel block
+      for (int tap = 0; tap < 27; ++tap) {
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        const int chunk = tap * F + blk;
+        bf16x8 B[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          B[t] = __builtin_bit_cast(bf16x8,
+              *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
+        const uint16_t *base = halo + (((dz * kHY) + (w + dy)) * kHX + m + dx) * CC + 32 * b + 8 * kg;
+#pragma unroll
+        for (int r = 0; r < MT; ++r) {
+          const bf16x8 A = __builtin_bit_cast(bf16x8,
+              *reinterpret_cast<const uint4 *>(base + r * (kHX * kHY * CC)));
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+        }
       }
+    }
+    // section 2: the last 16 channels (when CC = 16 mod 32), tap pairs
+    if constexpr (CC % 32 == 16) {
+      const int tsel = kg >> 1;
+      const int cofs = 32 * B32 + 8 * (kg & 1);
+      for (int p = 0; p < 14; ++p) {
+        const int chunk = 27 * F + p;
+        const int tap = 2 * p + tsel;
+        const int tp = tap < 27 ? tap : 26;  // tap 27 reads a valid address, zeroed below
+        const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+        bf16x8 B[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const bf16x8 B = __builtin_bit_cast(bf16x8,
-            *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
+        for (int t = 0; t < NT; ++t)
+          B[t] = __builtin_bit_cast(bf16x8,
+              *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
+        const uint16_t *base = halo + (((dz * kHY) + (w + dy)) * kHX + m + dx) * CC + cofs;
 #pragma unroll
-        for (int r = 0; r < MT; ++r)
-          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], B, acc[r][t], 0, 0, 0);
+        for (int r = 0; r < MT; ++r) {
+          uint4 av = *reinterpret_cast<const uint4 *>(base + r * (kHX * kHY * CC));
+          if (tap >= 27)
+            av = make_uint4(0, 0, 0, 0);
+          const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+        }
       }
     }
   }
-  // section 2: the last 16 channels of tap pairs (2p, 2p+1); tap 27 is zero
-  if (a.cin % 32) {
-    const int ci0 = 32 * F + 8 * (kg & 1) - 8 * kg; // xoff already holds +8*kg
-    for (int p = 0; p < 14; ++p, ++chunk) {
-      const int tap = 2 * p + (kg >> 1);
-      const uint32_t off = xoff + (uint32_t)((tap_offset(tap, D) * a.ldx + ci0) * 2);
-      bf16x8 A[MT];
-#pragma unroll
-      for (int r = 0; r < MT; ++r) {
-        const bool ok = tap < 27 && ((mask[r] >> tap) & 1u);
-        const uint32_t o = ok ? off + r * rowStride : kMissOffset;
-        A[r] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(xr, o, 0, 0));
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const bf16x8 B = __builtin_bit_cast(bf16x8,
-            *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
-#pragma unroll
-        for (int r = 0; r < MT; ++r)
-          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], B, acc[r][t], 0, 0, 0);
-      }
-    }
-  }
-  // epilogue: lane holds D[voxel 4*kg + i of the row][channel 16t + m]
-  const size_t ob = (((size_t)n * D + z) * D + y0) * D + xb * 16 + 4 * kg;
+  // epilogue: lane holds D[voxel x0 + 4*kg + i][channel 16t + m] of row r
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int co = 16 * t + m;
     const float bias = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
     for (int r = 0; r < MT; ++r) {
+      const size_t ob = ((((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + 4 * kg);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        uint16_t *dst = a.y + (ob + (size_t)r * D + i) * a.ldy + a.yofs + co;
+        uint16_t *dst = a.y + (ob + i) * a.ldy + a.yofs + co;
         float val = acc[r][t][i] + bias;
         if (a.relu)
           val = fmaxf(val, 0.0f);
@@ -188,6 +228,45 @@ __global__ __launch_bounds__(256) void conv3_kernel(ConvArgs a)
           val += bf2f(*dst);
         *dst = f2bf(val);
       }
+    }
+  }
+}
+
+// First layer: one live input channel (the FP32 volume), 27 taps x 16
+// output channels on the VALU; the taps are rounded to bfloat16 as the
+// staged input of the other layers is.  w0 = [16][27] bfloat16-exact floats.
+__global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict__ in,
+    const float *__restrict__ w0, const float *__restrict__ b0, uint16_t *__restrict__ y, int D,
+    size_t nvox)
+{
+  __shared__ float sw[16 * 27 + 16];
+  for (int i = threadIdx.x; i < 16 * 27; i += blockDim.x)
+    sw[i] = w0[i];
+  if (threadIdx.x < 16)
+    sw[16 * 27 + threadIdx.x] = b0[threadIdx.x];
+  __syncthreads();
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int x = v % D, yy = (v / D) % D, z = (v / ((size_t)D * D)) % D;
+    float tv[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const bool ok = (unsigned)(z + dz) < (unsigned)D && (unsigned)(yy + dy) < (unsigned)D &&
+                      (unsigned)(x + dx) < (unsigned)D;
+      tv[t] = ok ? bf2f(f2bf(in[(long)v + ((long)dz * D + dy) * D + dx])) : 0.0f;
+    }
+    uint32_t *dst = reinterpret_cast<uint32_t *>(y + v * 16);
+#pragma unroll 1
+    for (int c2 = 0; c2 < 8; ++c2) {
+      const float *w0a = sw + (2 * c2) * 27, *w0b = w0a + 27;
+      float s0 = sw[16 * 27 + 2 * c2], s1 = sw[16 * 27 + 2 * c2 + 1];
+#pragma unroll
+      for (int t = 0; t < 27; ++t) {
+        s0 = fmaf(w0a[t], tv[t], s0);
+        s1 = fmaf(w0b[t], tv[t], s1);
+      }
+      dst[c2] = (uint32_t)f2bf(fmaxf(s0, 0.0f)) | ((uint32_t)f2bf(fmaxf(s1, 0.0f)) << 16);
     }
   }
 }
@@ -281,16 +360,25 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
   }
 }
 
-// bias gradient: db[c] += sum_v g[v][c]
-__global__ void bgrad_kernel(const uint16_t *g, int ldg, int gofs, int C, int nvox, float *db)
+// bias gradient: db[c] += sum_v g[v][c] (C divides 256)
+__global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t *g, int ldg, int gofs, int C,
+    int nvox, float *db)
 {
+  __shared__ float red[256];
   const int c = threadIdx.x % C;
   const int lanesPerC = blockDim.x / C;
   const int sub = threadIdx.x / C;
   float s = 0.0f;
   for (int v = blockIdx.x * lanesPerC + sub; v < nvox; v += gridDim.x * lanesPerC)
     s += bf2f(g[(size_t)v * ldg + gofs + c]);
-  atomicAdd(db + c, s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.0f;
+    for (int k = 0; k < lanesPerC; ++k)
+      t += red[k * C + threadIdx.x];
+    atomicAdd(db + threadIdx.x, t);
+  }
 }
 
 // g *= (y > 0) over a channel slice
@@ -394,46 +482,81 @@ __global__ void head_kernel(const uint16_t *h, const float *in, const float *w, 
   }
 }
 
-// MSE: loss += sum (out-t)^2 / N;  gy = 2 (out - t) / N
-__global__ void mse_kernel(const float *out, const float *t, float *gy, size_t nvox, float *loss)
+// Block-wide sum of NV per-thread values (256 threads); thread 0 receives the
+// totals in v[].
+template <int NV> __device__ __forceinline__ void block_sum(float (&v)[NV])
 {
-  float s = 0.0f;
+  __shared__ float red[4][NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    for (int o = 32; o > 0; o >>= 1)
+      v[k] += __shfl_xor(v[k], o);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      red[wv][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      v[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+}
+
+// MSE: loss += sum (out-t)^2 / N;  gy = 2 (out - t) / N
+__global__ __launch_bounds__(256) void mse_kernel(const float *out, const float *t, float *gy,
+    size_t nvox, float *loss)
+{
+  float s[1] = { 0.0f };
   const float inv = 1.0f / (float)nvox;
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
     const float d = out[v] - t[v];
-    s += d * d;
+    s[0] += d * d;
     gy[v] = 2.0f * d * inv;
   }
-  for (int o = 32; o > 0; o >>= 1)
-    s += __shfl_xor(s, o);
-  if ((threadIdx.x & 63) == 0)
-    atomicAdd(loss, s * inv);
+  block_sum(s);
+  if (threadIdx.x == 0)
+    atomicAdd(loss, s[0] * inv);
 }
 
 // head backward: gh[v][c] = gy[v] w[c]; gw[c] += sum gy h; gb += sum gy
-__global__ void head_bwd_kernel(const float *gy, const uint16_t *h, const float *w, uint16_t *gh,
-    float *gw, float *gb, size_t nvox)
+__global__ __launch_bounds__(256) void head_bwd_kernel(const float *gy, const uint16_t *h,
+    const float *w, uint16_t *gh, float *gw, float *gb, size_t nvox)
 {
-  float sw[16], sb = 0.0f;
+  float acc[17];
+  float wl[16];
+#pragma unroll
+  for (int c = 0; c < 17; ++c)
+    acc[c] = 0.0f;
+#pragma unroll
   for (int c = 0; c < 16; ++c)
-    sw[c] = 0.0f;
+    wl[c] = w[c];
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
     const float g = gy[v];
-    sb += g;
-    for (int c = 0; c < 16; ++c) {
-      sw[c] = fmaf(g, bf2f(h[v * 16 + c]), sw[c]);
-      gh[v * 16 + c] = f2bf(g * w[c]);
+    acc[16] += g;
+    const uint4 *hv = reinterpret_cast<const uint4 *>(h + v * 16);
+    uint4 *gv = reinterpret_cast<uint4 *>(gh + v * 16);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const uint4 hq = hv[q];
+      const uint32_t hw[4] = { hq.x, hq.y, hq.z, hq.w };
+      uint32_t ow[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int c = 8 * q + 2 * k;
+        acc[c] = fmaf(g, __uint_as_float(hw[k] << 16), acc[c]);
+        acc[c + 1] = fmaf(g, __uint_as_float(hw[k] & 0xffff0000u), acc[c + 1]);
+        ow[k] = (uint32_t)f2bf(g * wl[c]) | ((uint32_t)f2bf(g * wl[c + 1]) << 16);
+      }
+      gv[q] = make_uint4(ow[0], ow[1], ow[2], ow[3]);
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    sb += __shfl_xor(sb, o);
+  block_sum(acc);
+  if (threadIdx.x == 0) {
+    atomicAdd(gb, acc[16]);
+#pragma unroll
     for (int c = 0; c < 16; ++c)
-      sw[c] += __shfl_xor(sw[c], o);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(gb, sb);
-    for (int c = 0; c < 16; ++c)
-      atomicAdd(gw + c, sw[c]);
+      atomicAdd(gw + c, acc[c]);
   }
 }
 
@@ -463,6 +586,14 @@ __global__ void pack_weights_kernel(const float *W, int cin, int cout, uint16_t 
     wf[(size_t)co * kpf + kpos(cin, tap, ci)] = h;
     wb[(size_t)ci * kpb + kpos(cout, 26 - tap, co)] = h;
   }
+}
+
+// first-layer weights for conv_first_kernel: w0[co][tap] = bf16(W[co][tap][0])
+__global__ void pack_first_kernel(const float *W, int cin, float *w0)
+{
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if (i < 16 * 27)
+    w0[i] = bf2f(f2bf(W[(size_t)i * cin]));
 }
 
 } // namespace
@@ -495,6 +626,7 @@ struct Act {
 struct LhUnet {
   Conv c[10];
   float *hw = nullptr, *hb = nullptr, *ghw = nullptr, *ghb = nullptr;
+  float *w0 = nullptr;  // first layer, [16][27], bfloat16-exact
   float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
   int step = 0;
   hipStream_t stream = nullptr;
@@ -528,22 +660,91 @@ template <typename T> int dalloc(T **p, size_t n)
   return LH_OK;
 }
 
-// rows of 16 voxels per wave for the forward/backward-data kernel
-constexpr int kConvMT = 2;
+// Convolution input: channels [0, c0) from a0 (at half resolution when
+// half0), channels [c0, cin) from a1.
+struct ConvIn {
+  Act a0;
+  int half0 = 0;
+  int c0 = 0;  // 0: all channels from a0
+  Act a1;
+};
 
-template <int NT> void launch_conv(hipStream_t s, const ConvArgs &a)
+ConvIn plain_in(Act a)
 {
-  const size_t waves = (size_t)a.n * a.D * (a.D / kConvMT) * (a.D / 16);
-  conv3_kernel<NT, kConvMT><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(a);
+  ConvIn c;
+  c.a0 = a;
+  c.a1 = a;
+  return c;
 }
 
-int conv_fwd(hipStream_t s, const Conv &L, Act in, Act out, int D, int n, bool relu,
+template <int CC, int NT, int MT> int launch_lds(hipStream_t s, ConvLdsArgs &a)
+{
+  const size_t lds = (size_t)(MT + 2) * kHY * kHX * CC * sizeof(uint16_t);
+  static bool attr = false;
+  if (!attr) {
+    UN_HIP(hipFuncSetAttribute((const void *)conv3_lds_kernel<CC, NT, MT>,
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  if (a.D % MT)
+    return lh_fail(LH_E_ARG, "conv: D=%d is not a multiple of the z tile %d", a.D, MT);
+  const size_t tiles = (size_t)a.n * (a.D / 16) * (a.D / 4) * (a.D / MT);
+  a.tilesPerXcd = (tiles % 8 == 0) ? (int)(tiles / 8) : 0;
+  conv3_lds_kernel<CC, NT, MT><<<(unsigned)tiles, 256, lds, s>>>(a);
+  return LH_OK;
+}
+
+template <int CC> int launch_cout(hipStream_t s, ConvLdsArgs &a, int cout)
+{
+  switch (cout) {
+  case 16: return launch_lds<CC, 1, 8>(s, a);
+  case 32: return launch_lds<CC, 2, 8>(s, a);
+  case 48: return launch_lds<CC, 3, 4>(s, a);
+  case 64: return launch_lds<CC, 4, 4>(s, a);
+  case 96: return launch_lds<CC, 6, 4>(s, a);
+  default: return lh_fail(LH_E_ARG, "conv: unsupported output channels %d", cout);
+  }
+}
+
+int make_src(ConvSrc &sr, Act a, int half, int D, int n)
+{
+  const int Ds = half ? D / 2 : D;
+  const uint64_t bytes = (uint64_t)n * Ds * Ds * Ds * a.ld * 2;
+  if (bytes >= kMissOffset)
+    return lh_fail(LH_E_ARG, "conv: input of %llu bytes exceeds the 2 GiB buffer range",
+        (unsigned long long)bytes);
+  if (a.ld % 8 || a.ofs % 8)
+    return lh_fail(LH_E_ARG, "conv: channel stride/offset must be multiples of 8");
+  sr.p = a.p;
+  sr.ld = a.ld;
+  sr.ofs = a.ofs;
+  sr.half = half;
+  sr.bytes = (int)bytes;
+  return LH_OK;
+}
+
+int conv_fwd(hipStream_t s, const Conv &L, ConvIn in, Act out, int D, int n, bool relu,
     bool bias, bool accumulate, bool backward)
 {
-  ConvArgs a;
-  a.x = in.p;
-  a.ldx = in.ld;
-  a.xofs = in.ofs;
+  ConvLdsArgs a;
+  int cout;
+  if (!backward) {
+    a.w = L.wf; a.cin = L.cin; cout = L.cout; a.kp = L.kpf; a.bias = bias ? L.b : nullptr;
+  } else {
+    a.w = L.wb; a.cin = L.cout; cout = L.cin; a.kp = L.kpb; a.bias = nullptr;
+  }
+  if (D % 16 || D < 16)
+    return lh_fail(LH_E_ARG, "conv: D must be a multiple of 16 (got %d)", D);
+  if (a.cin % 16)
+    return lh_fail(LH_E_ARG, "conv: input channels must be a multiple of 16");
+  a.c0 = in.c0 ? in.c0 : a.cin;
+  if (a.c0 % 8 || a.c0 > a.cin || (in.half0 && D % 2))
+    return lh_fail(LH_E_ARG, "conv: bad channel split %d", a.c0);
+  int rc = make_src(a.s0, in.a0, in.half0, D, n);
+  if (rc == LH_OK)
+    rc = make_src(a.s1, in.c0 ? in.a1 : in.a0, 0, D, n);
+  if (rc != LH_OK)
+    return rc;
   a.y = out.p;
   a.ldy = out.ld;
   a.yofs = out.ofs;
@@ -551,30 +752,24 @@ int conv_fwd(hipStream_t s, const Conv &L, Act in, Act out, int D, int n, bool r
   a.n = n;
   a.relu = relu;
   a.accumulate = accumulate;
-  if (!backward) {
-    a.w = L.wf; a.cin = L.cin; a.cout = L.cout; a.kp = L.kpf; a.bias = bias ? L.b : nullptr;
-  } else {
-    a.w = L.wb; a.cin = L.cout; a.cout = L.cin; a.kp = L.kpb; a.bias = nullptr;
+  switch (a.cin) {
+  case 16: rc = launch_cout<16>(s, a, cout); break;
+  case 48: rc = launch_cout<48>(s, a, cout); break;
+  case 32:
+  case 64:
+  case 96: rc = launch_cout<32>(s, a, cout); break;
+  default: return lh_fail(LH_E_ARG, "conv: unsupported input channels %d", a.cin);
   }
-  if (D % 16 || D < 16)
-    return lh_fail(LH_E_ARG, "conv: D must be a multiple of 16 (got %d)", D);
-  if (a.cin % 16 || a.ldx % 8 || a.xofs % 8)
-    return lh_fail(LH_E_ARG, "conv: input channels/stride must be multiples of 16/8");
-  const uint64_t xBytes = (uint64_t)n * D * D * D * a.ldx * 2;
-  if (xBytes >= kMissOffset)
-    return lh_fail(LH_E_ARG, "conv: input of %llu bytes exceeds the 2 GiB buffer range",
-        (unsigned long long)xBytes);
-  a.xBytes = (int)xBytes;
-  switch (a.cout) {
-  case 16: launch_conv<1>(s, a); break;
-  case 32: launch_conv<2>(s, a); break;
-  case 48: launch_conv<3>(s, a); break;
-  case 64: launch_conv<4>(s, a); break;
-  case 96: launch_conv<6>(s, a); break;
-  default: return lh_fail(LH_E_ARG, "conv: unsupported output channels %d", a.cout);
-  }
+  if (rc != LH_OK)
+    return rc;
   UN_HIP(hipGetLastError());
   return LH_OK;
+}
+
+int conv_fwd(hipStream_t s, const Conv &L, Act in, Act out, int D, int n, bool relu, bool bias,
+    bool accumulate, bool backward)
+{
+  return conv_fwd(s, L, plain_in(in), out, D, n, relu, bias, accumulate, backward);
 }
 
 int conv_wgrad(hipStream_t s, const Conv &L, Act x, Act g, int D, int n)
@@ -615,6 +810,8 @@ int pack_all(LhUnet *u)
         L.cout, L.wf, L.kpf, L.wb, L.kpb);
     UN_HIP(hipGetLastError());
   }
+  pack_first_kernel<<<2, 256, 0, u->stream>>>(u->c[0].W, u->c[0].cin, u->w0);
+  UN_HIP(hipGetLastError());
   return LH_OK;
 }
 
@@ -652,8 +849,11 @@ int ensure_ws(LhUnet *u, int n, int D)
   return LH_OK;
 }
 
-// forward through the whole net; activations stay in the workspace
-int forward(LhUnet *u, const float *in, int n, int D)
+// forward through the whole net; activations stay in the workspace.  The
+// decoder reads its upsampled inputs in place; with `training` the
+// concatenations (and the padded first-layer input) are also materialised
+// for the weight gradients.
+int forward(LhUnet *u, const float *in, int n, int D, bool training)
 {
   if (D % 64)
     return lh_fail(LH_E_ARG, "U-Net input edge must be a multiple of 64 (got %d)", D);
@@ -663,24 +863,38 @@ int forward(LhUnet *u, const float *in, int n, int D)
   hipStream_t s = u->stream;
   const int D1 = D, D2 = D / 2, D3 = D / 4;
   const int n1 = n * D1 * D1 * D1, n2 = n * D2 * D2 * D2, n3 = n * D3 * D3 * D3;
-  pack_input_kernel<<<grid_for((size_t)n1 * 16), 256, 0, s>>>(in, u->x16, (size_t)n1);
   auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
-#define FW(L, i, o, Dd, nv) if ((rc = conv_fwd(s, u->c[L], i, o, Dd, n, true, true, false, false)) != LH_OK) return rc
-  FW(0, A(u->x16, 16), A(u->a1, 16), D1, n1);
-  FW(1, A(u->a1, 16), A(u->cat1, 48, 32), D1, n1);
+  if (training)
+    pack_input_kernel<<<grid_for((size_t)n1 * 16), 256, 0, s>>>(in, u->x16, (size_t)n1);
+  conv_first_kernel<<<grid_for((size_t)n1, 256, 8192), 256, 0, s>>>(in, u->w0, u->c[0].b, u->a1,
+      D1, (size_t)n1);
+#define FW(L, i, o, Dd) if ((rc = conv_fwd(s, u->c[L], i, o, Dd, n, true, true, false, false)) != LH_OK) return rc
+  FW(1, A(u->a1, 16), A(u->cat1, 48, 32), D1);
   pool_kernel<<<grid_for((size_t)n2 * 16), 256, 0, s>>>(u->cat1, 48, 32, u->p1, 16, 0, 16, D2, n);
-  FW(2, A(u->p1, 16), A(u->a2, 32), D2, n2);
-  FW(3, A(u->a2, 32), A(u->cat2, 96, 64), D2, n2);
+  FW(2, A(u->p1, 16), A(u->a2, 32), D2);
+  FW(3, A(u->a2, 32), A(u->cat2, 96, 64), D2);
   pool_kernel<<<grid_for((size_t)n3 * 32), 256, 0, s>>>(u->cat2, 96, 64, u->p2, 32, 0, 32, D3, n);
-  FW(4, A(u->p2, 32), A(u->a3, 64), D3, n3);
-  FW(5, A(u->a3, 64), A(u->b3, 64), D3, n3);
-  up_kernel<<<grid_for((size_t)n2 * 64), 256, 0, s>>>(u->b3, 64, 0, u->cat2, 96, 0, 64, D3, n);
-  FW(6, A(u->cat2, 96), A(u->d2a, 32), D2, n2);
-  FW(7, A(u->d2a, 32), A(u->d2b, 32), D2, n2);
-  up_kernel<<<grid_for((size_t)n1 * 32), 256, 0, s>>>(u->d2b, 32, 0, u->cat1, 48, 0, 32, D2, n);
-  FW(8, A(u->cat1, 48), A(u->d1a, 16), D1, n1);
-  FW(9, A(u->d1a, 16), A(u->d1b, 16), D1, n1);
+  FW(4, A(u->p2, 32), A(u->a3, 64), D3);
+  FW(5, A(u->a3, 64), A(u->b3, 64), D3);
+  ConvIn c2;
+  c2.a0 = A(u->b3, 64);
+  c2.half0 = 1;
+  c2.c0 = 64;
+  c2.a1 = A(u->cat2, 96, 64);
+  FW(6, c2, A(u->d2a, 32), D2);
+  FW(7, A(u->d2a, 32), A(u->d2b, 32), D2);
+  ConvIn c1;
+  c1.a0 = A(u->d2b, 32);
+  c1.half0 = 1;
+  c1.c0 = 32;
+  c1.a1 = A(u->cat1, 48, 32);
+  FW(8, c1, A(u->d1a, 16), D1);
+  FW(9, A(u->d1a, 16), A(u->d1b, 16), D1);
 #undef FW
+  if (training) {
+    up_kernel<<<grid_for((size_t)n2 * 64), 256, 0, s>>>(u->b3, 64, 0, u->cat2, 96, 0, 64, D3, n);
+    up_kernel<<<grid_for((size_t)n1 * 32), 256, 0, s>>>(u->d2b, 32, 0, u->cat1, 48, 0, 32, D2, n);
+  }
   head_kernel<<<grid_for((size_t)n1), 256, 0, s>>>(u->d1b, in, u->hw, u->hb, u->out, (size_t)n1);
   UN_HIP(hipGetLastError());
   return LH_OK;
@@ -713,7 +927,8 @@ int backward(LhUnet *u, const float *in, int n, int D)
   };
 #define WG(L, x, g, Dd) if ((rc = conv_wgrad(s, u->c[L], x, g, Dd, n)) != LH_OK) return rc
 #define BD(L, g, o, Dd, nv, acc) if ((rc = conv_fwd(s, u->c[L], g, o, Dd, n, false, false, acc, true)) != LH_OK) return rc
-  head_bwd_kernel<<<grid_for(v1), 256, 0, s>>>(u->gy, u->d1b, u->hw, u->g_d1b, u->ghw, u->ghb, v1);
+  head_bwd_kernel<<<grid_for(v1, 256, 1024), 256, 0, s>>>(u->gy, u->d1b, u->hw, u->g_d1b, u->ghw,
+      u->ghb, v1);
   relub(u->g_d1b, u->d1b, 16, 0, 16, n1);
   WG(9, A(u->d1a, 16), A(u->g_d1b, 16), D1);
   BD(9, A(u->g_d1b, 16), A(u->g_d1a, 16), D1, n1, false);
@@ -808,7 +1023,8 @@ int lhUnetCreate(LhUnet **out, uint64_t seed)
     UN_HIP(hipMemcpy(L.W, h.data(), nw * sizeof(float), hipMemcpyHostToDevice));
   }
   int rc;
-  if ((rc = dalloc(&u->hw, 16)) || (rc = dalloc(&u->hb, 1)) || (rc = dalloc(&u->ghw, 16)) ||
+  if ((rc = dalloc(&u->w0, 16 * 27)) || (rc = dalloc(&u->hw, 16)) || (rc = dalloc(&u->hb, 1)) ||
+      (rc = dalloc(&u->ghw, 16)) ||
       (rc = dalloc(&u->ghb, 1)) || (rc = dalloc(&u->mhw, 16)) || (rc = dalloc(&u->vhw, 16)) ||
       (rc = dalloc(&u->mhb, 1)) || (rc = dalloc(&u->vhb, 1)))
     return rc;
@@ -834,7 +1050,7 @@ int lhUnetDestroy(LhUnet *u)
     for (void *p : { (void *)L.W, (void *)L.b, (void *)L.gW, (void *)L.gb, (void *)L.mW, (void *)L.vW,
              (void *)L.mb, (void *)L.vb, (void *)L.wf, (void *)L.wb })
       (void)hipFree(p);
-  for (void *p : { (void *)u->hw, (void *)u->hb, (void *)u->ghw, (void *)u->ghb, (void *)u->mhw,
+  for (void *p : { (void *)u->w0, (void *)u->hw, (void *)u->hb, (void *)u->ghw, (void *)u->ghb, (void *)u->mhw,
            (void *)u->vhw, (void *)u->mhb, (void *)u->vhb })
     (void)hipFree(p);
   for (void *p : u->ws)
@@ -919,7 +1135,7 @@ int lhUnetForward(LhUnet *u, const float *dIn, float *dOut, int n, int D)
 {
   if (!u || !dIn || !dOut || n < 1)
     return lh_fail(LH_E_ARG, "lhUnetForward: bad arguments");
-  int rc = forward(u, dIn, n, D);
+  int rc = forward(u, dIn, n, D, false);
   if (rc != LH_OK)
     return rc;
   const size_t nv = (size_t)n * D * D * D;
@@ -935,12 +1151,12 @@ int lhUnetTrainStep(LhUnet *u, const float *dIn, const float *dTarget, int n, in
 {
   if (!u || !dIn || !dTarget || n < 1)
     return lh_fail(LH_E_ARG, "lhUnetTrainStep: bad arguments");
-  int rc = forward(u, dIn, n, D);
+  int rc = forward(u, dIn, n, D, true);
   if (rc != LH_OK)
     return rc;
   const size_t nv = (size_t)n * D * D * D;
   UN_HIP(hipMemsetAsync(u->loss, 0, sizeof(float), u->stream));
-  mse_kernel<<<grid_for(nv), 256, 0, u->stream>>>(u->out, dTarget, u->gy, nv, u->loss);
+  mse_kernel<<<grid_for(nv, 256, 1024), 256, 0, u->stream>>>(u->out, dTarget, u->gy, nv, u->loss);
   UN_HIP(hipGetLastError());
   if ((rc = backward(u, dIn, n, D)) != LH_OK)
     return rc;

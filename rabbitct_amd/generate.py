@@ -18,10 +18,10 @@ def _p(a):
 
 def generate_into(ctx: "lolahip.Context", geom: ScanGeometry, phantom: EllipsoidPhantom,
                   first_slot: int = 0, first_id: int = 1, views=None, border: bool = True,
-                  want_host: bool = False, chunk: int = 16):
+                  want_host: bool = False, chunk: int = 16, ids=None):
     """Generate the views ``views`` (default all) of ``geom`` into slots
-    first_slot.. with ids first_id..; returns the float32 images if
-    ``want_host``."""
+    first_slot.. with ids first_id.. (or the 1-based ``ids``, one per view);
+    returns the float32 images if ``want_host``."""
     lib = lolahip.load_library()
     views = np.arange(geom.n_views) if views is None else np.asarray(views)
     S, Minv, axis = geom.rays()
@@ -30,6 +30,8 @@ def generate_into(ctx: "lolahip.Context", geom: ScanGeometry, phantom: Ellipsoid
     dbeta = math.radians(geom.arc_deg) / geom.n_views
     ell = np.ascontiguousarray(phantom.params, dtype=np.float64)
     host = (np.empty((len(views), geom.height, geom.width), np.float32) if want_host else None)
+    ids = (np.arange(first_id, first_id + len(views)) if ids is None else np.asarray(ids))
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
     for k0 in range(0, len(views), chunk):
         sel = views[k0:k0 + chunk]
         n = len(sel)
@@ -39,7 +41,7 @@ def generate_into(ctx: "lolahip.Context", geom: ScanGeometry, phantom: Ellipsoid
         ax = np.ascontiguousarray(axis[sel])
         pk = np.ascontiguousarray(parker[sel])
         out = host[k0:k0 + n] if want_host else None
-        rc = lib.lhGenerateViews(ctx._h, first_slot + k0, n, first_id + k0, _p(m), _p(s),
+        rc = lib.lhGenerateViews(ctx._h, first_slot + k0, n, _p(ids[k0:k0 + n]), _p(m), _p(s),
                                  _p(mi), _p(ax), _p(pk), ell.shape[0], _p(ell), tau, dbeta,
                                  1 if border else 0, None if out is None else _p(out))
         lolahip._check(rc, "lhGenerateViews")

@@ -124,7 +124,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhEffectiveUpdates.argtypes = [P, u32]
     lib.lhEffectiveUpdates.restype = ctypes.c_uint64
     lib.lhLastError.restype = ctypes.c_char_p
-    lib.lhGenerateViews.argtypes = [P, u32, u32, i32, P, P, P, P, P, u32, P, ctypes.c_double,
+    lib.lhGenerateViews.argtypes = [P, u32, u32, P, P, P, P, P, P, u32, P, ctypes.c_double,
                                     ctypes.c_double, i32, P]
     lib.lhSetLaunch.argtypes = [P, i32, u32, u32, i32]
     lib.lhCopyCalibration.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(f32)]
