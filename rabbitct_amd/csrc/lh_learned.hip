@@ -56,14 +56,19 @@ __device__ __forceinline__ float trilinear(const float *__restrict__ vol, int L,
 
 // Line integral of the volume along the ray of detector pixel
 // (i*stride, j*stride): midpoint rule with step h (mm) between the entry and
-// exit of the box that holds the trilinear support.  geo = S[3], Minv[9].
-__global__ void fp_kernel(const float *__restrict__ vol, int L, float R, float O, int Wd,
-    int stride, const double *__restrict__ geo, float h, float *__restrict__ out)
+// exit of the box that holds the trilinear support.  kSegLanes lanes share a
+// ray, each taking every kSegLanes-th sample; their partial sums are reduced
+// with cross-lane shuffles.  geo = S[3], Minv[9].
+constexpr int kSegLanes = 16;
+
+__global__ __launch_bounds__(256) void fp_kernel(const float *__restrict__ vol, int L, float R,
+    float O, int Wd, int Hd, int stride, const double *__restrict__ geo, float h,
+    float *__restrict__ out)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= Wd)
-    return;
+  const int ray = (blockIdx.x * blockDim.x + threadIdx.x) / kSegLanes;
+  const int seg = threadIdx.x % kSegLanes;
+  const int i = ray % Wd, j = ray / Wd;
+  const bool live = j < Hd;
   const double *S = geo, *Mi = geo + 3;
   double d[3];
   for (int r = 0; r < 3; ++r)
@@ -92,7 +97,7 @@ __global__ void fp_kernel(const float *__restrict__ vol, int L, float R, float O
     }
   }
   float acc = 0.0f;
-  if (t1 > t0) {
+  if (live && t1 > t0) {
     // the chord of the box is at most sqrt(3) (L + 1) R
     const double nmax = 2.0 * (double)(L + 1) * (double)R / (double)h + 2.0;
     const int n = (int)fmin(ceil((t1 - t0) / (double)h), nmax);
@@ -101,11 +106,14 @@ __global__ void fp_kernel(const float *__restrict__ vol, int L, float R, float O
     const float px = (float)(p0[0] + ts * dp[0]), py = (float)(p0[1] + ts * dp[1]),
                 pz = (float)(p0[2] + ts * dp[2]);
     const float sx = (float)(h * dp[0]), sy = (float)(h * dp[1]), sz = (float)(h * dp[2]);
-    for (int k = 0; k < n; ++k)
+    for (int k = seg; k < n; k += kSegLanes)
       acc += trilinear(vol, L, fmaf((float)k, sx, px), fmaf((float)k, sy, py),
           fmaf((float)k, sz, pz));
   }
-  out[(size_t)j * Wd + i] = acc * h;
+  for (int o = kSegLanes / 2; o > 0; o >>= 1)
+    acc += __shfl_xor(acc, o, kSegLanes);
+  if (live && seg == 0)
+    out[(size_t)j * Wd + i] = acc * h;
 }
 
 // sums[0] += sum (a - b)^2, sums[1] += sum b^2
@@ -189,8 +197,9 @@ int lhForwardProject(const float *dVol, int L, float R, float O, int W, int H, i
   if (e == hipSuccess)
     e = hipMemcpy(dGeo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice);
   for (uint32_t v = 0; e == hipSuccess && v < count; ++v) {
-    fp_kernel<<<dim3((Wd + 127) / 128, Hd), 128>>>(dVol, L, R, O, Wd, stride, dGeo + 12 * v, h,
-        dOut + (size_t)v * Wd * Hd);
+    const size_t lanes = (size_t)Wd * Hd * kSegLanes;
+    fp_kernel<<<(unsigned)((lanes + 255) / 256), 256>>>(dVol, L, R, O, Wd, Hd, stride,
+        dGeo + 12 * v, h, dOut + (size_t)v * Wd * Hd);
     e = hipGetLastError();
   }
   if (e == hipSuccess)

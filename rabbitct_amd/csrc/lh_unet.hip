@@ -92,12 +92,18 @@ struct ConvLdsArgs {
   int D, n, cin, kp;
   int relu, accumulate;
   int tilesPerXcd;    // > 0: tiles split into 8 contiguous ranges, one per XCD
+  // fused 1x1x1 head (16 output channels only): hout[v] = hin[v] + hb +
+  // sum_c hw[c] y[v][c]; y itself is written only when non-null
+  const float *hin, *hw, *hb;
+  float *hout;
 };
 
 constexpr int kHX = 18, kHY = 6;
 
+// Stages the halo of input channels [cbase, cbase + CC) (CC = 32 or 16) as
+// [hz][hy][hx][CC] bfloat16.
 template <int CC, int MT>
-__device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, int pass, int n,
+__device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, int cbase, int n,
     int x0, int y0, int z0)
 {
   constexpr int Q = CC / 8;                // 16-byte units per voxel
@@ -109,7 +115,7 @@ __device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, 
     const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
     const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
                     (unsigned)gz < (unsigned)D;
-    const int c = pass * CC + 8 * q;
+    const int c = cbase + 8 * q;
     const bool first = c < a.c0;
     const ConvSrc &sr = first ? a.s0 : a.s1;
     const int cc = first ? c : c - a.c0;
@@ -126,10 +132,31 @@ __device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, 
   }
 }
 
-template <int CC, int NT, int MT>
+template <int NT>
+__device__ __forceinline__ void load_b(bf16x8 (&B)[NT], const uint16_t *wrow, size_t wtile,
+    int chunk)
+{
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+    B[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
+}
+
+// LDS bytes of a launch: the larger of the 32-channel halo and the output
+// transpose buffer (4 waves x MT rows x 16 voxels x NT*16 channels).
+template <int NT, int MT> constexpr size_t conv_lds_bytes()
+{
+  const size_t halo = (size_t)(MT + 2) * kHY * kHX * 32 * 2;
+  const size_t outb = (size_t)4 * MT * 16 * NT * 16 * 2;
+  return halo > outb ? halo : outb;
+}
+
+// Input channels are staged in passes: F = cin/32 passes of 32 channels
+// (one tap per K chunk), then with TAIL a pass of the last 16 channels (tap
+// pairs per K chunk), matching the weight packing.
+template <int NT, int MT, bool TAIL>
 __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
 {
-  extern __shared__ __align__(16) uint16_t halo[];
+  extern __shared__ __align__(16) uint16_t lds[];
   const int D = a.D;
   const int txN = D / 16, tyN = D / 4, tzN = D / MT;
   int tile = blockIdx.x;
@@ -145,129 +172,211 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
   const int m = lane & 15, kg = lane >> 4;
   const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
   const size_t wtile = (size_t)16 * a.kp;
-  const int F = a.cin / 32;                  // 32-channel blocks in section 1
+  const int F = a.cin / 32;
   f32x4 acc[MT][NT];
 #pragma unroll
   for (int r = 0; r < MT; ++r)
 #pragma unroll
     for (int t = 0; t < NT; ++t)
       acc[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const int passes = a.cin / CC;
-  for (int pass = 0; pass < passes; ++pass) {
+  constexpr int RS32 = kHX * kHY * 32, RS16 = kHX * kHY * 16; // z-row strides
+  for (int pass = 0; pass < F; ++pass) {
     if (pass)
       __syncthreads();
-    stage_halo<CC, MT>(a, halo, pass, n, x0, y0, z0);
+    stage_halo<32, MT>(a, lds, 32 * pass, n, x0, y0, z0);
     __syncthreads();
-    // section 1: 32-channel blocks of this pass, one tap per chunk
-    constexpr int B32 = CC / 32;
+    bf16x8 Bn[NT];
+    load_b(Bn, wrow, wtile, pass);
+    for (int tap = 0; tap < 27; ++tap) {
+      bf16x8 B[NT];
 #pragma unroll
-    for (int b = 0; b < B32; ++b) {
-      const int blk = pass * B32 + b;        // global 32-channel block
-      for (int tap = 0; tap < 27; ++tap) {
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        const int chunk = tap * F + blk;
-        bf16x8 B[NT];
+      for (int t = 0; t < NT; ++t)
+        B[t] = Bn[t];
+      if (tap + 1 < 27)
+        load_b(Bn, wrow, wtile, (tap + 1) * F + pass);
+      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+      const uint16_t *base = lds + ((dz * kHY + w + dy) * kHX + m + dx) * 32 + 8 * kg;
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-          B[t] = __builtin_bit_cast(bf16x8,
-              *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
-        const uint16_t *base = halo + (((dz * kHY) + (w + dy)) * kHX + m + dx) * CC + 32 * b + 8 * kg;
-#pragma unroll
-        for (int r = 0; r < MT; ++r) {
-          const bf16x8 A = __builtin_bit_cast(bf16x8,
-              *reinterpret_cast<const uint4 *>(base + r * (kHX * kHY * CC)));
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
-        }
-      }
-    }
-    // section 2: the last 16 channels (when CC = 16 mod 32), tap pairs
-    if constexpr (CC % 32 == 16) {
-      const int tsel = kg >> 1;
-      const int cofs = 32 * B32 + 8 * (kg & 1);
-      for (int p = 0; p < 14; ++p) {
-        const int chunk = 27 * F + p;
-        const int tap = 2 * p + tsel;
-        const int tp = tap < 27 ? tap : 26;  // tap 27 reads a valid address, zeroed below
-        const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
-        bf16x8 B[NT];
+      for (int r = 0; r < MT; ++r) {
+        const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + r * RS32));
 #pragma unroll
         for (int t = 0; t < NT; ++t)
-          B[t] = __builtin_bit_cast(bf16x8,
-              *reinterpret_cast<const uint4 *>(wrow + t * wtile + chunk * 32));
-        const uint16_t *base = halo + (((dz * kHY) + (w + dy)) * kHX + m + dx) * CC + cofs;
-#pragma unroll
-        for (int r = 0; r < MT; ++r) {
-          uint4 av = *reinterpret_cast<const uint4 *>(base + r * (kHX * kHY * CC));
-          if (tap >= 27)
-            av = make_uint4(0, 0, 0, 0);
-          const bf16x8 A = __builtin_bit_cast(bf16x8, av);
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
-        }
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
       }
     }
   }
-  // epilogue: lane holds D[voxel x0 + 4*kg + i][channel 16t + m] of row r
+  if constexpr (TAIL) {
+    if (F)
+      __syncthreads();
+    stage_halo<16, MT>(a, lds, 32 * F, n, x0, y0, z0);
+    __syncthreads();
+    const int tsel = kg >> 1, cofs = 8 * (kg & 1);
+    bf16x8 Bn[NT];
+    load_b(Bn, wrow, wtile, 27 * F);
+    for (int p = 0; p < 14; ++p) {
+      bf16x8 B[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        B[t] = Bn[t];
+      if (p + 1 < 14)
+        load_b(Bn, wrow, wtile, 27 * F + p + 1);
+      const int tap = 2 * p + tsel;
+      const int tp = tap < 27 ? tap : 26;    // tap 27 reads a valid address, zeroed below
+      const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+      const uint16_t *base = lds + ((dz * kHY + w + dy) * kHX + m + dx) * 16 + cofs;
+#pragma unroll
+      for (int r = 0; r < MT; ++r) {
+        uint4 av = *reinterpret_cast<const uint4 *>(base + r * RS16);
+        if (tap >= 27)
+          av = make_uint4(0, 0, 0, 0);
+        const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: bias, ReLU, bfloat16 into an LDS transpose buffer (this wave's
+  // rows: [MT][16 voxels][COUT]), then 16-byte stores of 8 channels
+  constexpr int COUT = NT * 16;
+  __syncthreads();
+  uint16_t *ob = lds + (size_t)w * MT * 16 * COUT;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int co = 16 * t + m;
     const float bias = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
-    for (int r = 0; r < MT; ++r) {
-      const size_t ob = ((((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + 4 * kg);
+    for (int r = 0; r < MT; ++r)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        uint16_t *dst = a.y + (ob + i) * a.ldy + a.yofs + co;
         float val = acc[r][t][i] + bias;
         if (a.relu)
           val = fmaxf(val, 0.0f);
-        if (a.accumulate)
-          val += bf2f(*dst);
-        *dst = f2bf(val);
+        ob[(r * 16 + 4 * kg + i) * COUT + co] = f2bf(val);
+      }
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): this wave's LDS writes are done
+  __builtin_amdgcn_wave_barrier();
+  constexpr int UNITS = MT * 16 * COUT / 8;  // 16-byte units of this wave
+  if (a.y) {
+#pragma unroll
+    for (int u0 = 0; u0 < UNITS; u0 += 64) {
+      const int u = u0 + lane;
+      if (UNITS % 64 == 0 || u < UNITS) {
+        const int c8 = u % (COUT / 8), vx = (u / (COUT / 8)) % 16, r = u / (COUT / 8 * 16);
+        uint4 val = *reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * COUT + 8 * c8);
+        const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
+        uint4 *dst = reinterpret_cast<uint4 *>(a.y + vo * a.ldy + a.yofs + 8 * c8);
+        if (a.accumulate) {
+          const uint4 old = *dst;
+          const uint32_t nv[4] = { val.x, val.y, val.z, val.w };
+          const uint32_t ov[4] = { old.x, old.y, old.z, old.w };
+          uint32_t rv[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            rv[k] = (uint32_t)f2bf(__uint_as_float(nv[k] << 16) + __uint_as_float(ov[k] << 16)) |
+                    ((uint32_t)f2bf(__uint_as_float(nv[k] & 0xffff0000u) +
+                                    __uint_as_float(ov[k] & 0xffff0000u)) << 16);
+          val = make_uint4(rv[0], rv[1], rv[2], rv[3]);
+        }
+        *dst = val;
+      }
+    }
+  }
+  if constexpr (COUT == 16) {
+    if (a.hout) {
+      // one voxel per lane: its 16 channels (bfloat16) dotted with the head
+#pragma unroll
+      for (int v0 = 0; v0 < MT * 16; v0 += 64) {
+        const int vv = v0 + lane;
+        if ((MT * 16) % 64 == 0 || vv < MT * 16) {
+          const int vx = vv % 16, r = vv / 16;
+          const uint4 *hv = reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * 16);
+          const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
+          float sum = a.hin[vo] + a.hb[0];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const uint4 hq = hv[q];
+            const uint32_t hw4[4] = { hq.x, hq.y, hq.z, hq.w };
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              sum = fmaf(a.hw[8 * q + 2 * k], __uint_as_float(hw4[k] << 16), sum);
+              sum = fmaf(a.hw[8 * q + 2 * k + 1], __uint_as_float(hw4[k] & 0xffff0000u), sum);
+            }
+          }
+          a.hout[vo] = sum;
+        }
       }
     }
   }
 }
 
-// First layer: one live input channel (the FP32 volume), 27 taps x 16
-// output channels on the VALU; the taps are rounded to bfloat16 as the
-// staged input of the other layers is.  w0 = [16][27] bfloat16-exact floats.
+// First layer: one live input channel (the FP32 volume, rounded to bfloat16),
+// K = 27 taps (padded to 32), 16 output channels: one MFMA per 16 voxels.
+// Block tile 16 x 4 x 8 voxels as in conv3_lds_kernel; the input halo
+// (18 x 6 x 10 voxels) is staged in LDS as bfloat16 and each lane gathers its
+// 8 taps from it.  w0 = bfloat16 [16][32] (taps 27..31 zero).
 __global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict__ in,
-    const float *__restrict__ w0, const float *__restrict__ b0, uint16_t *__restrict__ y, int D,
-    size_t nvox)
+    const uint16_t *__restrict__ w0, const float *__restrict__ b0, uint16_t *__restrict__ y,
+    int D, int tilesPerXcd)
 {
-  __shared__ float sw[16 * 27 + 16];
-  for (int i = threadIdx.x; i < 16 * 27; i += blockDim.x)
-    sw[i] = w0[i];
-  if (threadIdx.x < 16)
-    sw[16 * 27 + threadIdx.x] = b0[threadIdx.x];
+  constexpr int MT = 8;
+  __shared__ uint16_t halo[kHX * kHY * (MT + 2)];
+  __shared__ __align__(16) uint16_t obuf[4 * MT * 16 * 16];
+  const int txN = D / 16, tyN = D / 4, tzN = D / MT;
+  int tile = blockIdx.x;
+  if (tilesPerXcd > 0)
+    tile = (blockIdx.x & 7) * tilesPerXcd + (blockIdx.x >> 3);
+  const int tx = tile % txN;
+  int rest = tile / txN;
+  const int ty = rest % tyN;
+  rest /= tyN;
+  const int tz = rest % tzN, n = rest / tzN;
+  const int x0 = tx * 16, y0 = ty * 4, z0 = tz * MT;
+  for (int i = threadIdx.x; i < kHX * kHY * (MT + 2); i += 256) {
+    const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+    const bool ok = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+                    (unsigned)gz < (unsigned)D;
+    halo[i] = ok ? f2bf(in[(((size_t)n * D + gz) * D + gy) * D + gx]) : (uint16_t)0;
+  }
   __syncthreads();
-  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox;
-       v += (size_t)gridDim.x * blockDim.x) {
-    const int x = v % D, yy = (v / D) % D, z = (v / ((size_t)D * D)) % D;
-    float tv[27];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(w0 + m * 32 + 8 * kg));
+  int hoff[8];
 #pragma unroll
-    for (int t = 0; t < 27; ++t) {
-      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bool ok = (unsigned)(z + dz) < (unsigned)D && (unsigned)(yy + dy) < (unsigned)D &&
-                      (unsigned)(x + dx) < (unsigned)D;
-      tv[t] = ok ? bf2f(f2bf(in[(long)v + ((long)dz * D + dy) * D + dx])) : 0.0f;
-    }
-    uint32_t *dst = reinterpret_cast<uint32_t *>(y + v * 16);
-#pragma unroll 1
-    for (int c2 = 0; c2 < 8; ++c2) {
-      const float *w0a = sw + (2 * c2) * 27, *w0b = w0a + 27;
-      float s0 = sw[16 * 27 + 2 * c2], s1 = sw[16 * 27 + 2 * c2 + 1];
+  for (int j = 0; j < 8; ++j) {
+    const int tap = 8 * kg + j;
+    const int tp = tap < 27 ? tap : 0;
+    hoff[j] = ((tp / 9) * kHY + w + (tp / 3) % 3) * kHX + m + tp % 3;
+  }
+  uint16_t *ob = obuf + w * MT * 16 * 16;
+  const float bias = b0[m];
 #pragma unroll
-      for (int t = 0; t < 27; ++t) {
-        s0 = fmaf(w0a[t], tv[t], s0);
-        s1 = fmaf(w0b[t], tv[t], s1);
-      }
-      dst[c2] = (uint32_t)f2bf(fmaxf(s0, 0.0f)) | ((uint32_t)f2bf(fmaxf(s1, 0.0f)) << 16);
-    }
+  for (int r = 0; r < MT; ++r) {
+    uint16_t av[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      av[j] = (8 * kg + j < 27) ? halo[hoff[j] + r * kHX * kHY] : (uint16_t)0;
+    const uint4 a4 = make_uint4((uint32_t)av[0] | ((uint32_t)av[1] << 16),
+        (uint32_t)av[2] | ((uint32_t)av[3] << 16), (uint32_t)av[4] | ((uint32_t)av[5] << 16),
+        (uint32_t)av[6] | ((uint32_t)av[7] << 16));
+    const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a4), B,
+        f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      ob[(r * 16 + 4 * kg + i) * 16 + m] = f2bf(fmaxf(acc[i] + bias, 0.0f));
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int u0 = 0; u0 < MT * 16 * 2; u0 += 64) {
+    const int u = u0 + lane;
+    const int c8 = u & 1, vx = (u >> 1) % 16, r = u >> 5;
+    const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
+    *reinterpret_cast<uint4 *>(y + vo * 16 + 8 * c8) =
+        *reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * 16 + 8 * c8);
   }
 }
 
@@ -279,7 +388,7 @@ __global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict
 // The 3*COT*CIT tiles are spread over the 4 waves; partial sums are added to
 // dW atomically.
 struct WgradArgs {
-  const uint16_t *x; int ldx, xofs;
+  ConvSrc s0, s1; int c0;   // input channels [0, c0) from s0 (maybe half resolution), rest s1
   const uint16_t *g; int ldg, gofs;
   float *dw;           // [cout][27][cin]
   int D, nrows;        // rows = n*D*D x-rows of D voxels
@@ -324,8 +433,18 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a)
         const int xv = rem % kSeg, dx = rem / kSeg;
         const int sx = x0 + xv + dx - 1;
         uint16_t val = 0;
-        if (rowOk && xv < seg && (unsigned)sx < (unsigned)D)
-          val = a.x[(xBase + sx) * a.ldx + a.xofs + ci];
+        if (rowOk && xv < seg && (unsigned)sx < (unsigned)D) {
+          const bool first = ci < a.c0;
+          const ConvSrc &sr = first ? a.s0 : a.s1;
+          const int cc = first ? ci : ci - a.c0;
+          if (sr.half) {
+            const int Dh = D >> 1;
+            val = sr.p[((((size_t)n * Dh + (sz >> 1)) * Dh + (sy >> 1)) * Dh + (sx >> 1)) * sr.ld +
+                       sr.ofs + cc];
+          } else {
+            val = sr.p[(xBase + sx) * sr.ld + sr.ofs + cc];
+          }
+        }
         xT[(dx * CIN + ci) * kSegP + xv] = val;
       }
       __syncthreads();
@@ -382,33 +501,45 @@ __global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t *g, int ldg, 
 }
 
 // g *= (y > 0) over a channel slice
-__global__ void relu_bwd_kernel(uint16_t *g, const uint16_t *y, int ld, int ofs, int C, int nvox)
+__global__ void relu_bwd_kernel(uint16_t *g, int ldg, int gofs, const uint16_t *y, int ldy,
+    int yofs, int C, int nvox)
 {
   const size_t n = (size_t)nvox * C;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const size_t v = i / C, c = i % C;
-    const size_t p = v * ld + ofs + c;
-    if (!(bf2f(y[p]) > 0.0f))
-      g[p] = 0;
+    if (!(bf2f(y[v * ldy + yofs + c]) > 0.0f))
+      g[v * ldg + gofs + c] = 0;
   }
 }
 
-// 2x2x2 average pooling, channel slices in and out
+// 2x2x2 average pooling, channel slices in and out (C, strides and offsets
+// multiples of 8: one thread per 8 channels of an output voxel)
 __global__ void pool_kernel(const uint16_t *x, int ldx, int xofs, uint16_t *y, int ldy, int yofs,
     int C, int Do, int n)
 {
-  const size_t total = (size_t)n * Do * Do * Do * C;
+  const int C8 = C / 8;
+  const size_t total = (size_t)n * Do * Do * Do * C8;
   const int Di = 2 * Do;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    const int c = i % C;
-    size_t v = i / C;
+    const int c = 8 * (int)(i % C8);
+    const size_t v = i / C8;
     const int xo = v % Do, yo = (v / Do) % Do, zo = (v / (Do * Do)) % Do, b = v / ((size_t)Do * Do * Do);
-    float s = 0.0f;
+    float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
     for (int q = 0; q < 8; ++q) {
       const size_t vi = (((size_t)b * Di + 2 * zo + (q >> 2)) * Di + 2 * yo + ((q >> 1) & 1)) * Di + 2 * xo + (q & 1);
-      s += bf2f(x[vi * ldx + xofs + c]);
+      const uint4 u = *reinterpret_cast<const uint4 *>(x + vi * ldx + xofs + c);
+      const uint32_t w4[4] = { u.x, u.y, u.z, u.w };
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[2 * k] += __uint_as_float(w4[k] << 16);
+        s[2 * k + 1] += __uint_as_float(w4[k] & 0xffff0000u);
+      }
     }
-    y[v * ldy + yofs + c] = f2bf(0.125f * s);
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = (uint32_t)f2bf(0.125f * s[2 * k]) | ((uint32_t)f2bf(0.125f * s[2 * k + 1]) << 16);
+    *reinterpret_cast<uint4 *>(y + v * ldy + yofs + c) = make_uint4(o[0], o[1], o[2], o[3]);
   }
 }
 
@@ -425,21 +556,6 @@ __global__ void pool_bwd_kernel(const uint16_t *gy, int ldy, int yofs, uint16_t 
     const size_t vo = (((size_t)b * Do + zi / 2) * Do + yi / 2) * Do + xi / 2;
     uint16_t *dst = gx + v * ldx + xofs + c;
     *dst = f2bf(bf2f(*dst) + 0.125f * bf2f(gy[vo * ldy + yofs + c]));
-  }
-}
-
-// nearest 2x upsampling into a channel slice
-__global__ void up_kernel(const uint16_t *x, int ldx, int xofs, uint16_t *y, int ldy, int yofs,
-    int C, int Di, int n)
-{
-  const int Do = 2 * Di;
-  const size_t total = (size_t)n * Do * Do * Do * C;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    const int c = i % C;
-    const size_t v = i / C;
-    const int xo = v % Do, yo = (v / Do) % Do, zo = (v / (Do * Do)) % Do, b = v / ((size_t)Do * Do * Do);
-    const size_t vi = (((size_t)b * Di + zo / 2) * Di + yo / 2) * Di + xo / 2;
-    y[v * ldy + yofs + c] = x[vi * ldx + xofs + c];
   }
 }
 
@@ -588,12 +704,13 @@ __global__ void pack_weights_kernel(const float *W, int cin, int cout, uint16_t 
   }
 }
 
-// first-layer weights for conv_first_kernel: w0[co][tap] = bf16(W[co][tap][0])
-__global__ void pack_first_kernel(const float *W, int cin, float *w0)
+// first-layer weights for conv_first_kernel: w0[co][tap] = bf16(W[co][tap][0]),
+// rows of 32 (taps 27..31 stay zero)
+__global__ void pack_first_kernel(const float *W, int cin, uint16_t *w0)
 {
   const int i = threadIdx.x + blockIdx.x * blockDim.x;
   if (i < 16 * 27)
-    w0[i] = bf2f(f2bf(W[(size_t)i * cin]));
+    w0[(i / 27) * 32 + i % 27] = f2bf(W[(size_t)i * cin]);
 }
 
 } // namespace
@@ -626,14 +743,14 @@ struct Act {
 struct LhUnet {
   Conv c[10];
   float *hw = nullptr, *hb = nullptr, *ghw = nullptr, *ghb = nullptr;
-  float *w0 = nullptr;  // first layer, [16][27], bfloat16-exact
+  uint16_t *w0 = nullptr;  // first layer, bfloat16 [16][32]
   float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
   int step = 0;
   hipStream_t stream = nullptr;
   // workspace for one (n, D)
   int wsN = 0, wsD = 0;
   std::vector<void *> ws;
-  uint16_t *x16, *a1, *cat1, *p1, *a2, *cat2, *p2, *a3, *b3, *d2a, *d2b, *d1a, *d1b;
+  uint16_t *x16, *a1, *e1, *p1, *a2, *e2, *p2, *a3, *b3, *d2a, *d2b, *d1a, *d1b;
   uint16_t *g_a1, *g_cat1, *g_p1, *g_a2, *g_cat2, *g_p2, *g_a3, *g_b3, *g_d2a, *g_d2b, *g_d1a,
       *g_d1b;
   float *out, *gy, *loss;
@@ -677,31 +794,35 @@ ConvIn plain_in(Act a)
   return c;
 }
 
-template <int CC, int NT, int MT> int launch_lds(hipStream_t s, ConvLdsArgs &a)
+template <int NT, int MT, bool TAIL> int launch_lds(hipStream_t s, ConvLdsArgs &a)
 {
-  const size_t lds = (size_t)(MT + 2) * kHY * kHX * CC * sizeof(uint16_t);
+  constexpr size_t ldsMax = conv_lds_bytes<NT, MT>();
   static bool attr = false;
   if (!attr) {
-    UN_HIP(hipFuncSetAttribute((const void *)conv3_lds_kernel<CC, NT, MT>,
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    UN_HIP(hipFuncSetAttribute((const void *)conv3_lds_kernel<NT, MT, TAIL>,
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsMax));
     attr = true;
   }
+  // a 16-channel input stages a 16-channel halo only
+  const size_t halo16 = (size_t)(MT + 2) * kHY * kHX * 16 * 2;
+  const size_t outb = (size_t)4 * MT * 16 * NT * 16 * 2;
+  const size_t lds = (a.cin >= 32) ? ldsMax : std::max(halo16, outb);
   if (a.D % MT)
     return lh_fail(LH_E_ARG, "conv: D=%d is not a multiple of the z tile %d", a.D, MT);
   const size_t tiles = (size_t)a.n * (a.D / 16) * (a.D / 4) * (a.D / MT);
   a.tilesPerXcd = (tiles % 8 == 0) ? (int)(tiles / 8) : 0;
-  conv3_lds_kernel<CC, NT, MT><<<(unsigned)tiles, 256, lds, s>>>(a);
+  conv3_lds_kernel<NT, MT, TAIL><<<(unsigned)tiles, 256, lds, s>>>(a);
   return LH_OK;
 }
 
-template <int CC> int launch_cout(hipStream_t s, ConvLdsArgs &a, int cout)
+template <bool TAIL> int launch_cout(hipStream_t s, ConvLdsArgs &a, int cout)
 {
   switch (cout) {
-  case 16: return launch_lds<CC, 1, 8>(s, a);
-  case 32: return launch_lds<CC, 2, 8>(s, a);
-  case 48: return launch_lds<CC, 3, 4>(s, a);
-  case 64: return launch_lds<CC, 4, 4>(s, a);
-  case 96: return launch_lds<CC, 6, 4>(s, a);
+  case 16: return launch_lds<1, 8, TAIL>(s, a);
+  case 32: return launch_lds<2, 8, TAIL>(s, a);
+  case 48: return launch_lds<3, 4, TAIL>(s, a);
+  case 64: return launch_lds<4, 4, TAIL>(s, a);
+  case 96: return launch_lds<6, 4, TAIL>(s, a);
   default: return lh_fail(LH_E_ARG, "conv: unsupported output channels %d", cout);
   }
 }
@@ -723,10 +844,21 @@ int make_src(ConvSrc &sr, Act a, int half, int D, int n)
   return LH_OK;
 }
 
+struct HeadFuse {
+  const float *in = nullptr, *w = nullptr, *b = nullptr;
+  float *out = nullptr;
+};
+
 int conv_fwd(hipStream_t s, const Conv &L, ConvIn in, Act out, int D, int n, bool relu,
-    bool bias, bool accumulate, bool backward)
+    bool bias, bool accumulate, bool backward, HeadFuse head = HeadFuse())
 {
   ConvLdsArgs a;
+  a.hin = head.in;
+  a.hw = head.w;
+  a.hb = head.b;
+  a.hout = head.out;
+  if (head.out && (backward ? L.cin : L.cout) != 16)
+    return lh_fail(LH_E_ARG, "conv: the fused head needs 16 output channels");
   int cout;
   if (!backward) {
     a.w = L.wf; a.cin = L.cin; cout = L.cout; a.kp = L.kpf; a.bias = bias ? L.b : nullptr;
@@ -752,14 +884,10 @@ int conv_fwd(hipStream_t s, const Conv &L, ConvIn in, Act out, int D, int n, boo
   a.n = n;
   a.relu = relu;
   a.accumulate = accumulate;
-  switch (a.cin) {
-  case 16: rc = launch_cout<16>(s, a, cout); break;
-  case 48: rc = launch_cout<48>(s, a, cout); break;
-  case 32:
-  case 64:
-  case 96: rc = launch_cout<32>(s, a, cout); break;
-  default: return lh_fail(LH_E_ARG, "conv: unsupported input channels %d", a.cin);
-  }
+  if (a.cin > 96 || (out.p && ((out.ld % 8) || (out.ofs % 8))))
+    return lh_fail(LH_E_ARG, "conv: unsupported channels (in %d, out stride %d offset %d)", a.cin,
+        out.ld, out.ofs);
+  rc = (a.cin % 32) ? launch_cout<true>(s, a, cout) : launch_cout<false>(s, a, cout);
   if (rc != LH_OK)
     return rc;
   UN_HIP(hipGetLastError());
@@ -772,10 +900,15 @@ int conv_fwd(hipStream_t s, const Conv &L, Act in, Act out, int D, int n, bool r
   return conv_fwd(s, L, plain_in(in), out, D, n, relu, bias, accumulate, backward);
 }
 
-int conv_wgrad(hipStream_t s, const Conv &L, Act x, Act g, int D, int n)
+int conv_wgrad(hipStream_t s, const Conv &L, ConvIn x, Act g, int D, int n)
 {
   WgradArgs a;
-  a.x = x.p; a.ldx = x.ld; a.xofs = x.ofs;
+  a.c0 = x.c0 ? x.c0 : L.cin;
+  int rc = make_src(a.s0, x.a0, x.half0, D, n);
+  if (rc == LH_OK)
+    rc = make_src(a.s1, x.c0 ? x.a1 : x.a0, 0, D, n);
+  if (rc != LH_OK)
+    return rc;
   a.g = g.p; a.ldg = g.ld; a.gofs = g.ofs;
   a.dw = L.gW;
   a.D = D;
@@ -803,6 +936,11 @@ int conv_wgrad(hipStream_t s, const Conv &L, Act x, Act g, int D, int n)
   return LH_OK;
 }
 
+int conv_wgrad(hipStream_t s, const Conv &L, Act x, Act g, int D, int n)
+{
+  return conv_wgrad(s, L, plain_in(x), g, D, n);
+}
+
 int pack_all(LhUnet *u)
 {
   for (auto &L : u->c) {
@@ -825,8 +963,8 @@ int ensure_ws(LhUnet *u, int n, int D)
   u->wsN = u->wsD = 0;
   const size_t v1 = (size_t)n * D * D * D, v2 = v1 / 8, v3 = v2 / 8;
   struct { uint16_t **p; size_t n; } bufs[] = {
-    { &u->x16, v1 * 16 }, { &u->a1, v1 * 16 }, { &u->cat1, v1 * 48 }, { &u->p1, v2 * 16 },
-    { &u->a2, v2 * 32 }, { &u->cat2, v2 * 96 }, { &u->p2, v3 * 32 }, { &u->a3, v3 * 64 },
+    { &u->x16, v1 * 16 }, { &u->a1, v1 * 16 }, { &u->e1, v1 * 16 }, { &u->p1, v2 * 16 },
+    { &u->a2, v2 * 32 }, { &u->e2, v2 * 32 }, { &u->p2, v3 * 32 }, { &u->a3, v3 * 64 },
     { &u->b3, v3 * 64 }, { &u->d2a, v2 * 32 }, { &u->d2b, v2 * 32 }, { &u->d1a, v1 * 16 },
     { &u->d1b, v1 * 16 }, { &u->g_a1, v1 * 16 }, { &u->g_cat1, v1 * 48 }, { &u->g_p1, v2 * 16 },
     { &u->g_a2, v2 * 32 }, { &u->g_cat2, v2 * 96 }, { &u->g_p2, v3 * 32 }, { &u->g_a3, v3 * 64 },
@@ -849,10 +987,27 @@ int ensure_ws(LhUnet *u, int n, int D)
   return LH_OK;
 }
 
+// Input of the first convolution of decoder level `lev` (2: up(b3) ++ e2,
+// 1: up(d2b) ++ e1), read in place.
+ConvIn dec_in(LhUnet *u, int lev)
+{
+  ConvIn c;
+  c.half0 = 1;
+  if (lev == 2) {
+    c.a0.p = u->b3; c.a0.ld = 64; c.a0.ofs = 0; c.c0 = 64;
+    c.a1.p = u->e2; c.a1.ld = 32; c.a1.ofs = 0;
+  } else {
+    c.a0.p = u->d2b; c.a0.ld = 32; c.a0.ofs = 0; c.c0 = 32;
+    c.a1.p = u->e1; c.a1.ld = 16; c.a1.ofs = 0;
+  }
+  return c;
+}
+
 // forward through the whole net; activations stay in the workspace.  The
-// decoder reads its upsampled inputs in place; with `training` the
-// concatenations (and the padded first-layer input) are also materialised
-// for the weight gradients.
+// decoder reads its upsampled inputs and the skip tensors in place (no
+// concatenation is materialised).  Inference fuses the head into the last
+// convolution; `training` keeps every activation the backward pass reads
+// (and the padded first-layer input for its weight gradient).
 int forward(LhUnet *u, const float *in, int n, int D, bool training)
 {
   if (D % 64)
@@ -866,36 +1021,42 @@ int forward(LhUnet *u, const float *in, int n, int D, bool training)
   auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
   if (training)
     pack_input_kernel<<<grid_for((size_t)n1 * 16), 256, 0, s>>>(in, u->x16, (size_t)n1);
-  conv_first_kernel<<<grid_for((size_t)n1, 256, 8192), 256, 0, s>>>(in, u->w0, u->c[0].b, u->a1,
-      D1, (size_t)n1);
+  {
+    const size_t tiles = (size_t)n * (D1 / 16) * (D1 / 4) * (D1 / 8);
+    conv_first_kernel<<<(unsigned)tiles, 256, 0, s>>>(in, u->w0, u->c[0].b, u->a1, D1,
+        tiles % 8 == 0 ? (int)(tiles / 8) : 0);
+  }
 #define FW(L, i, o, Dd) if ((rc = conv_fwd(s, u->c[L], i, o, Dd, n, true, true, false, false)) != LH_OK) return rc
-  FW(1, A(u->a1, 16), A(u->cat1, 48, 32), D1);
-  pool_kernel<<<grid_for((size_t)n2 * 16), 256, 0, s>>>(u->cat1, 48, 32, u->p1, 16, 0, 16, D2, n);
+  FW(1, A(u->a1, 16), A(u->e1, 16), D1);
+  pool_kernel<<<grid_for((size_t)n2 * 2), 256, 0, s>>>(u->e1, 16, 0, u->p1, 16, 0, 16, D2, n);
   FW(2, A(u->p1, 16), A(u->a2, 32), D2);
-  FW(3, A(u->a2, 32), A(u->cat2, 96, 64), D2);
-  pool_kernel<<<grid_for((size_t)n3 * 32), 256, 0, s>>>(u->cat2, 96, 64, u->p2, 32, 0, 32, D3, n);
+  FW(3, A(u->a2, 32), A(u->e2, 32), D2);
+  pool_kernel<<<grid_for((size_t)n3 * 4), 256, 0, s>>>(u->e2, 32, 0, u->p2, 32, 0, 32, D3, n);
   FW(4, A(u->p2, 32), A(u->a3, 64), D3);
   FW(5, A(u->a3, 64), A(u->b3, 64), D3);
-  ConvIn c2;
-  c2.a0 = A(u->b3, 64);
-  c2.half0 = 1;
-  c2.c0 = 64;
-  c2.a1 = A(u->cat2, 96, 64);
-  FW(6, c2, A(u->d2a, 32), D2);
+  FW(6, dec_in(u, 2), A(u->d2a, 32), D2);
   FW(7, A(u->d2a, 32), A(u->d2b, 32), D2);
-  ConvIn c1;
-  c1.a0 = A(u->d2b, 32);
-  c1.half0 = 1;
-  c1.c0 = 32;
-  c1.a1 = A(u->cat1, 48, 32);
-  FW(8, c1, A(u->d1a, 16), D1);
-  FW(9, A(u->d1a, 16), A(u->d1b, 16), D1);
+  FW(8, dec_in(u, 1), A(u->d1a, 16), D1);
 #undef FW
   if (training) {
-    up_kernel<<<grid_for((size_t)n2 * 64), 256, 0, s>>>(u->b3, 64, 0, u->cat2, 96, 0, 64, D3, n);
-    up_kernel<<<grid_for((size_t)n1 * 32), 256, 0, s>>>(u->d2b, 32, 0, u->cat1, 48, 0, 32, D2, n);
+    rc = conv_fwd(s, u->c[9], A(u->d1a, 16), A(u->d1b, 16), D1, n, true, true, false, false);
+    if (rc != LH_OK)
+      return rc;
+    head_kernel<<<grid_for((size_t)n1), 256, 0, s>>>(u->d1b, in, u->hw, u->hb, u->out, (size_t)n1);
+  } else {
+    HeadFuse h;
+    h.in = in;
+    h.w = u->hw;
+    h.b = u->hb;
+    h.out = u->out;
+    Act none;
+    none.p = nullptr;
+    none.ld = 16;
+    none.ofs = 0;
+    rc = conv_fwd(s, u->c[9], plain_in(A(u->d1a, 16)), none, D1, n, true, true, false, false, h);
+    if (rc != LH_OK)
+      return rc;
   }
-  head_kernel<<<grid_for((size_t)n1), 256, 0, s>>>(u->d1b, in, u->hw, u->hb, u->out, (size_t)n1);
   UN_HIP(hipGetLastError());
   return LH_OK;
 }
@@ -922,45 +1083,46 @@ int backward(LhUnet *u, const float *in, int n, int D)
   if (rc != LH_OK)
     return rc;
   auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
-  auto relub = [&](uint16_t *g, const uint16_t *y, int ld, int ofs, int C, int nv) {
-    relu_bwd_kernel<<<grid_for((size_t)nv * C), 256, 0, s>>>(g, y, ld, ofs, C, nv);
+  // g (channel slice of a gradient buffer) *= (y > 0)
+  auto relub = [&](uint16_t *g, int ldg, int gofs, const uint16_t *y, int ldy, int C, int nv) {
+    relu_bwd_kernel<<<grid_for((size_t)nv * C), 256, 0, s>>>(g, ldg, gofs, y, ldy, 0, C, nv);
   };
 #define WG(L, x, g, Dd) if ((rc = conv_wgrad(s, u->c[L], x, g, Dd, n)) != LH_OK) return rc
 #define BD(L, g, o, Dd, nv, acc) if ((rc = conv_fwd(s, u->c[L], g, o, Dd, n, false, false, acc, true)) != LH_OK) return rc
   head_bwd_kernel<<<grid_for(v1, 256, 1024), 256, 0, s>>>(u->gy, u->d1b, u->hw, u->g_d1b, u->ghw,
       u->ghb, v1);
-  relub(u->g_d1b, u->d1b, 16, 0, 16, n1);
+  relub(u->g_d1b, 16, 0, u->d1b, 16, 16, n1);
   WG(9, A(u->d1a, 16), A(u->g_d1b, 16), D1);
   BD(9, A(u->g_d1b, 16), A(u->g_d1a, 16), D1, n1, false);
-  relub(u->g_d1a, u->d1a, 16, 0, 16, n1);
-  WG(8, A(u->cat1, 48), A(u->g_d1a, 16), D1);
+  relub(u->g_d1a, 16, 0, u->d1a, 16, 16, n1);
+  WG(8, dec_in(u, 1), A(u->g_d1a, 16), D1);
   BD(8, A(u->g_d1a, 16), A(u->g_cat1, 48), D1, n1, false);
   up_bwd_kernel<<<grid_for(v2 * 32), 256, 0, s>>>(u->g_cat1, 48, 0, u->g_d2b, 32, 0, 32, D2, n);
-  relub(u->g_d2b, u->d2b, 32, 0, 32, n2);
+  relub(u->g_d2b, 32, 0, u->d2b, 32, 32, n2);
   WG(7, A(u->d2a, 32), A(u->g_d2b, 32), D2);
   BD(7, A(u->g_d2b, 32), A(u->g_d2a, 32), D2, n2, false);
-  relub(u->g_d2a, u->d2a, 32, 0, 32, n2);
-  WG(6, A(u->cat2, 96), A(u->g_d2a, 32), D2);
+  relub(u->g_d2a, 32, 0, u->d2a, 32, 32, n2);
+  WG(6, dec_in(u, 2), A(u->g_d2a, 32), D2);
   BD(6, A(u->g_d2a, 32), A(u->g_cat2, 96), D2, n2, false);
   up_bwd_kernel<<<grid_for((size_t)n3 * 64), 256, 0, s>>>(u->g_cat2, 96, 0, u->g_b3, 64, 0, 64, D3, n);
-  relub(u->g_b3, u->b3, 64, 0, 64, n3);
+  relub(u->g_b3, 64, 0, u->b3, 64, 64, n3);
   WG(5, A(u->a3, 64), A(u->g_b3, 64), D3);
   BD(5, A(u->g_b3, 64), A(u->g_a3, 64), D3, n3, false);
-  relub(u->g_a3, u->a3, 64, 0, 64, n3);
+  relub(u->g_a3, 64, 0, u->a3, 64, 64, n3);
   WG(4, A(u->p2, 32), A(u->g_a3, 64), D3);
   BD(4, A(u->g_a3, 64), A(u->g_p2, 32), D3, n3, false);
   pool_bwd_kernel<<<grid_for(v2 * 32), 256, 0, s>>>(u->g_p2, 32, 0, u->g_cat2, 96, 64, 32, D2, n);
-  relub(u->g_cat2, u->cat2, 96, 64, 32, n2);
+  relub(u->g_cat2, 96, 64, u->e2, 32, 32, n2);
   WG(3, A(u->a2, 32), A(u->g_cat2, 96, 64), D2);
   BD(3, A(u->g_cat2, 96, 64), A(u->g_a2, 32), D2, n2, false);
-  relub(u->g_a2, u->a2, 32, 0, 32, n2);
+  relub(u->g_a2, 32, 0, u->a2, 32, 32, n2);
   WG(2, A(u->p1, 16), A(u->g_a2, 32), D2);
   BD(2, A(u->g_a2, 32), A(u->g_p1, 16), D2, n2, false);
   pool_bwd_kernel<<<grid_for(v1 * 16), 256, 0, s>>>(u->g_p1, 16, 0, u->g_cat1, 48, 32, 16, D1, n);
-  relub(u->g_cat1, u->cat1, 48, 32, 16, n1);
+  relub(u->g_cat1, 48, 32, u->e1, 16, 16, n1);
   WG(1, A(u->a1, 16), A(u->g_cat1, 48, 32), D1);
   BD(1, A(u->g_cat1, 48, 32), A(u->g_a1, 16), D1, n1, false);
-  relub(u->g_a1, u->a1, 16, 0, 16, n1);
+  relub(u->g_a1, 16, 0, u->a1, 16, 16, n1);
   WG(0, A(u->x16, 16), A(u->g_a1, 16), D1);
 #undef WG
 #undef BD
@@ -1023,7 +1185,7 @@ int lhUnetCreate(LhUnet **out, uint64_t seed)
     UN_HIP(hipMemcpy(L.W, h.data(), nw * sizeof(float), hipMemcpyHostToDevice));
   }
   int rc;
-  if ((rc = dalloc(&u->w0, 16 * 27)) || (rc = dalloc(&u->hw, 16)) || (rc = dalloc(&u->hb, 1)) ||
+  if ((rc = dalloc(&u->w0, 16 * 32)) || (rc = dalloc(&u->hw, 16)) || (rc = dalloc(&u->hb, 1)) ||
       (rc = dalloc(&u->ghw, 16)) ||
       (rc = dalloc(&u->ghb, 1)) || (rc = dalloc(&u->mhw, 16)) || (rc = dalloc(&u->vhw, 16)) ||
       (rc = dalloc(&u->mhb, 1)) || (rc = dalloc(&u->vhb, 1)))
