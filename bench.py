@@ -42,7 +42,7 @@ sys.path.insert(0, ROOT)
 
 from rabbitct_amd import autotune, lolahip, rct  # noqa: E402
 from rabbitct_amd.generate import generate_into  # noqa: E402
-from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry  # noqa: E402
+from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry, voxel_size  # noqa: E402
 
 METRIC = "GUPS at L=512 plain vs fused/bf16; volumes/s sparse-view+CNN at fixed RMSE"
 HBM_PEAK = 8.0e12
@@ -208,6 +208,8 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a device-to-device copy of this size (HBM counter scale)")
     ap.add_argument("--tune-views", type=int, default=128)
+    ap.add_argument("--ellipsoids", type=int, default=10,
+                    help="phantom ellipsoids (30 for the L=1024 configuration)")
     ap.add_argument("--no-sparse-cnn", action="store_true",
                     help="skip the sparse-view + CNN volumes/s measurement at L=256")
     args = ap.parse_args()
@@ -217,7 +219,7 @@ def main():
         sys.exit(2)
 
     geom = ScanGeometry(n_views=args.views, seed=SEED).build()
-    phantom = EllipsoidPhantom(n_ell=10, seed=SEED).build()
+    phantom = EllipsoidPhantom(n_ell=args.ellipsoids, seed=SEED).build()
     log(f"geometry: L={args.L}, {args.views} views, {geom.width}x{geom.height}")
     ranges = lolahip.compute_line_ranges(args.L, geom.width, geom.height, geom.matrices, 16,
                                          device=True)
@@ -279,9 +281,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": f"synthetic: seeded 10-ellipsoid phantom (seed {SEED}), FDK-filtered analytic "
+        "data": f"synthetic: seeded {args.ellipsoids}-ellipsoid phantom (seed {SEED}), FDK-filtered analytic "
                 "projections generated on the GPU",
-        "config": {"workload": f"backprojection L={args.L} (R_L=0.5 mm), {args.views} views of "
+        "config": {"workload": f"backprojection L={args.L} (R_L={voxel_size(args.L)} mm), "
+                               f"{args.views} views of "
                                f"{geom.width}x{geom.height} float32, line clipping VECTORSIZE 16, "
                                f"{args.headline} b={vpp}",
                    "L": args.L, "views": args.views, "detector": [geom.width, geom.height],

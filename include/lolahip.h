@@ -167,9 +167,20 @@ int lhGetVolume(LhContext *ctx, float *host);
  * context's storage precision. */
 int lhLoadView(LhContext *ctx, uint32_t slot, int id, const double *matrix,
     const float *image);
-/* Same, with the image already in device memory (W x H floats). */
+/* Same, with the image already in device memory (W x H floats).  The image
+ * is read by kernels on the context's stream: it must stay unchanged until
+ * the stream has passed this call (work enqueued on the same stream after it
+ * may overwrite it). */
 int lhLoadViewDevice(LhContext *ctx, uint32_t slot, int id, const double *matrix,
     const float *dImage);
+
+/* lhLoadView in two halves, so the caller can fill the pinned buffer itself
+ * (e.g. with several threads).  Acquire returns the next of the context's
+ * pinned W x H float buffers, waiting only until its previous H2D copy has
+ * read it; commit enqueues its copy on the context's upload stream and the
+ * staging into `slot` on the compute stream, ordered by events. */
+int lhAcquireUpload(LhContext *ctx, float **hostBuffer);
+int lhCommitUpload(LhContext *ctx, uint32_t slot, int id, const double *matrix);
 
 /* Enqueues the backprojection of slots [firstSlot, firstSlot+count) on the
  * context's stream (asynchronous). */
@@ -182,6 +193,10 @@ int lhTimedBackproject(LhContext *ctx, uint32_t count, int reps, float *msPerRep
     float *msPerLaunch, uint32_t *launchesPerRep);
 
 int lhSynchronize(LhContext *ctx);
+
+/* enable != 0: lhBackproject replays one captured HIP graph per batch (slot
+ * range and launch shape), capturing it on first use. */
+int lhSetGraphs(LhContext *ctx, int enable);
 
 /* Changes the launch shape of later backprojections (kernel, views per
  * register pass, voxels per thread along x, LDS staging) without touching

@@ -35,6 +35,7 @@ def lib():
         _lib.orc_backproject_clipped.argtypes = [u32, f32, f32, u32, u32, P, P, P, P]
         _lib.orc_backproject_asm.argtypes = [u32, f32, f32, u32, u32, P, P, P, ctypes.c_int, P]
         _lib.orc_backproject_opt_f32.argtypes = [u32, f32, f32, u32, u32, P, P, P, P]
+        _lib.orc_backproject_lines.argtypes = [u32, f32, f32, u32, u32, P, P, P, u32, P, P, P]
         _lib.orc_line_ranges.argtypes = [u32, f32, f32, u32, u32, P, ctypes.c_int, P]
         _lib.orc_shadow.argtypes = [u32, f32, f32, u32, u32, u32, P, P]
         _lib.orc_hu_metric.argtypes = [P, ctypes.c_uint64, P, f32, f32, P, P]
@@ -91,6 +92,23 @@ def backproject_opt_f32(L, rL, oL, matrices, images, ranges):
         r = _c(ranges[n], np.int16)
         lib().orc_backproject_opt_f32(L, rL, oL, W, H, _p(a), _p(img), _p(r), _p(vol))
     return vol
+
+
+def backproject_lines(L, rL, oL, matrices, images, ranges, zs, ys):
+    """FP64 clipped backprojection of the (z, y) lines only: (n_lines, L)."""
+    matrices = _c(matrices, np.float64).reshape(-1, 12)
+    images = _c(images, np.float32)
+    zs = _c(zs, np.int32)
+    ys = _c(ys, np.int32)
+    N, H, W = images.shape
+    out = np.zeros((len(zs), L), dtype=np.float64)
+    for n in range(N):
+        a = _c(matrices[n], np.float64)
+        img = _c(images[n], np.float32)
+        r = _c(ranges[n], np.int16)
+        lib().orc_backproject_lines(L, rL, oL, W, H, _p(a), _p(img), _p(r), len(zs), _p(zs),
+                                    _p(ys), _p(out))
+    return out
 
 
 def line_ranges(L, rL, oL, W, H, matrices, vs):

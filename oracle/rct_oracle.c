@@ -11,6 +11,8 @@
  *                              forms (constant image, single pixel, 1/s^2 scaling)
  *   orc_backproject_clipped    golden vectors of the compiled LolaOPT/LolaOMP,
  *                              agreement with bunny inside the clip ranges
+ *   orc_backproject_lines      equality with orc_backproject_clipped on the
+ *                              same lines (it is that loop on a line subset)
  *   orc_backproject_asm        golden vectors of the compiled LolaASM (the
  *                              do-while "at least one vector" semantics)
  *   orc_line_ranges            byte equality with clip files written by the
@@ -112,6 +114,30 @@ void orc_backproject_clipped(uint32_t L, float rL, float oL, uint32_t W, uint32_
         project(a, wx, wy, wz, &u, &v, &w);
         vol[(z * L + y) * L + x] += bilinear(img, u, v, W, H) / (w * w);
       }
+    }
+  }
+}
+
+/* The clipped FP64 form restricted to a list of (z, y) lines: out[k*L + x]
+ * += the view's contribution to voxel x of line (zs[k], ys[k]); for sampled
+ * checks at sizes where the whole volume is out of reach (L=1024).  ranges
+ * holds the view's L*L (start, stop) pairs. */
+void orc_backproject_lines(uint32_t L, float rL, float oL, uint32_t W, uint32_t H,
+    const double *a, const float *img, const int16_t *ranges, uint32_t nLines,
+    const int32_t *zs, const int32_t *ys, double *out)
+{
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < (long)nLines; k++) {
+    const long z = zs[k], y = ys[k];
+    const double wz = (double)oL + (double)z * (double)rL;
+    const double wy = (double)oL + (double)y * (double)rL;
+    const long start = ranges[2 * (z * L + y)];
+    const long stop  = ranges[2 * (z * L + y) + 1];
+    for (long x = start; x < stop; x++) {
+      const double wx = (double)oL + (double)x * (double)rL;
+      double u, v, w;
+      project(a, wx, wy, wz, &u, &v, &w);
+      out[k * L + x] += bilinear(img, u, v, W, H) / (w * w);
     }
   }
 }

@@ -29,6 +29,8 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cstdarg>
 #include <cstddef>
 #include <atomic>
@@ -634,20 +636,23 @@ __device__ __forceinline__ uint16_t to_f16(float f)
 // to the slot's view parameters.
 template <int PREC>
 __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
-    uint32_t rowBytes, const uint32_t *maxbits, ViewParams *vp)
+    uint32_t rowBytes, const uint32_t *maxbits, ViewParams vp, ViewParams *out)
 {
   const int m    = storage_exponent(PREC, *maxbits);
   const float is = ldexpf(1.0f, -2 * m);
   const int ei   = blockIdx.x * blockDim.x + threadIdx.x; // element column
   const int ej   = blockIdx.y;                            // element row
-  if (PREC != LH_PREC_F32 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-    const double k = ldexp(1.0, -m);
-    for (int c = 0; c < 12; ++c)
-      vp->a[c] *= k;
-    vp->du *= (float)k;
-    vp->dv *= (float)k;
-    vp->dw *= (float)k;
-    vp->scale = ldexpf(1.0f, 2 * m);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (PREC != LH_PREC_F32) {
+      const double k = ldexp(1.0, -m);
+      for (int c = 0; c < 12; ++c)
+        vp.a[c] *= k;
+      vp.du *= (float)k;
+      vp.dv *= (float)k;
+      vp.dw *= (float)k;
+      vp.scale = ldexpf(1.0f, 2 * m);
+    }
+    *out = vp;
   }
   auto pix = [&](int i, int j) -> float {
     return (i >= 0 && i < W && j >= 0 && j < H) ? img[(size_t)j * W + i] : 0.0f;
@@ -757,9 +762,16 @@ struct LhContext {
   int *dRanges       = nullptr;
   ViewParams *dViews = nullptr;
   uint8_t *dSlots    = nullptr;
-  float *dStage      = nullptr;     // W*H floats
   uint32_t *dMax     = nullptr;     // one per slot
-  float *hPinned     = nullptr;     // W*H floats, pinned
+  // host-upload ring: pinned host buffers (allocated on first use) and device
+  // staging buffers of W*H floats; the H2D copies run on upStream
+  hipStream_t upStream = nullptr;
+  int nRing = 0, ringNext = 0, ringAcquired = -1;
+  std::vector<float *> hRing, dRing;
+  std::vector<hipEvent_t> copyDone, stageDone;
+  // captured backprojection batches, by (first slot, count, kernel, run, staging)
+  bool graphs = false;
+  std::map<std::tuple<uint32_t, uint32_t, int, int, int>, hipGraphExec_t> graphCache;
   uint64_t slotBytes = 0;
   uint32_t rowBytes  = 0;
   uint32_t slotRecords = 0;
@@ -877,10 +889,20 @@ int lhCreate(const LhConfig *cfg, LhContext **out)
   c->baseOffset        = (uint32_t)(kBorder * c->rowBytes + kBorder * elem);
   LH_HIP(hipMalloc(&c->dSlots, c->slotBytes * cfg->nSlots));
   LH_HIP(hipMemsetAsync(c->dSlots, 0, c->slotBytes * cfg->nSlots, c->stream));
-  LH_HIP(hipMalloc(&c->dStage, (uint64_t)cfg->W * cfg->H * sizeof(float)));
   LH_HIP(hipMalloc(&c->dMax, cfg->nSlots * sizeof(uint32_t)));
-  LH_HIP(hipHostMalloc((void **)&c->hPinned, (uint64_t)cfg->W * cfg->H * sizeof(float),
-      hipHostMallocDefault));
+  // one ring entry per view of a register pass (2..32), so a whole batch can
+  // be in flight while the previous one is backprojected
+  LH_HIP(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
+  c->nRing = (int)std::min<uint32_t>(32u, std::max<uint32_t>(2u, cfg->viewsPerPass));
+  c->hRing.assign(c->nRing, nullptr);
+  c->dRing.assign(c->nRing, nullptr);
+  c->copyDone.assign(c->nRing, nullptr);
+  c->stageDone.assign(c->nRing, nullptr);
+  for (int k = 0; k < c->nRing; ++k) {
+    LH_HIP(hipMalloc(&c->dRing[k], (uint64_t)cfg->W * cfg->H * sizeof(float)));
+    LH_HIP(hipEventCreateWithFlags(&c->copyDone[k], hipEventDisableTiming));
+    LH_HIP(hipEventCreateWithFlags(&c->stageDone[k], hipEventDisableTiming));
+  }
   c->effPerView.assign(cfg->nViews, cfg->clip ? 0 : L * L * L);
   c->slotId.assign(cfg->nSlots, 0);
   c->boxR.assign(cfg->nSlots, 0);
@@ -898,11 +920,23 @@ int lhDestroy(LhContext *c)
   (void)hipFree(c->dRanges);
   (void)hipFree(c->dViews);
   (void)hipFree(c->dSlots);
-  (void)hipFree(c->dStage);
+  if (c->upStream)
+    (void)hipStreamSynchronize(c->upStream);
+  for (auto &kv : c->graphCache)
+    (void)hipGraphExecDestroy(kv.second);
   (void)hipFree(c->dMax);
   (void)hipFree(c->dBoxes);
-  if (c->hPinned)
-    (void)hipHostFree(c->hPinned);
+  for (int k = 0; k < c->nRing; ++k) {
+    (void)hipFree(c->dRing[k]);
+    if (c->hRing[k])
+      (void)hipHostFree(c->hRing[k]);
+    if (c->copyDone[k])
+      (void)hipEventDestroy(c->copyDone[k]);
+    if (c->stageDone[k])
+      (void)hipEventDestroy(c->stageDone[k]);
+  }
+  if (c->upStream)
+    (void)hipStreamDestroy(c->upStream);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1043,7 +1077,11 @@ int lhVolumeDevice(LhContext *c, float **dVol)
   return LH_OK;
 }
 
-static int load_view_common(LhContext *c, uint32_t slot, int id, const double *matrix)
+// Enqueues on the context stream: max-abs of the W x H image src (device),
+// then its conversion into slot `slot` with the view parameters of `matrix`
+// (passed by value to the staging kernel, which writes them to the slot).
+static int load_view_common(LhContext *c, uint32_t slot, int id, const double *matrix,
+    const float *src)
 {
   const LhConfig &g = c->cfg;
   if (slot >= g.nSlots || id < 1 || (uint32_t)id > g.nViews || !matrix)
@@ -1057,10 +1095,9 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
   vp.dw    = (float)(matrix[2] * (double)g.R_L);
   vp.id    = id;
   vp.scale = 1.0f;
-  LH_HIP(hipMemcpyAsync(c->dViews + slot, &vp, sizeof vp, hipMemcpyHostToDevice, c->stream));
   LH_HIP(hipMemsetAsync(c->dMax + slot, 0, sizeof(uint32_t), c->stream));
   const uint32_t n = g.W * g.H;
-  maxabs_kernel<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, c->stream>>>(c->dStage, n,
+  maxabs_kernel<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, c->stream>>>(src, n,
       c->dMax + slot);
   const int cols = g.precision == LH_PREC_F32 ? (int)g.W + 4 : (int)g.W + 3;
   const int rows = g.precision == LH_PREC_F32 ? (int)g.H + 4 : (int)g.H + 3;
@@ -1068,16 +1105,16 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
   uint8_t *dst = c->dSlots + (uint64_t)slot * c->slotBytes;
   switch (g.precision) {
   case LH_PREC_F32:
-    stage_kernel<LH_PREC_F32><<<grid, 256, 0, c->stream>>>(c->dStage, g.W, g.H, dst, c->rowBytes,
-        c->dMax + slot, c->dViews + slot);
+    stage_kernel<LH_PREC_F32><<<grid, 256, 0, c->stream>>>(src, g.W, g.H, dst, c->rowBytes,
+        c->dMax + slot, vp, c->dViews + slot);
     break;
   case LH_PREC_BF16:
-    stage_kernel<LH_PREC_BF16><<<grid, 256, 0, c->stream>>>(c->dStage, g.W, g.H, dst, c->rowBytes,
-        c->dMax + slot, c->dViews + slot);
+    stage_kernel<LH_PREC_BF16><<<grid, 256, 0, c->stream>>>(src, g.W, g.H, dst, c->rowBytes,
+        c->dMax + slot, vp, c->dViews + slot);
     break;
   default:
-    stage_kernel<LH_PREC_F16><<<grid, 256, 0, c->stream>>>(c->dStage, g.W, g.H, dst, c->rowBytes,
-        c->dMax + slot, c->dViews + slot);
+    stage_kernel<LH_PREC_F16><<<grid, 256, 0, c->stream>>>(src, g.W, g.H, dst, c->rowBytes,
+        c->dMax + slot, vp, c->dViews + slot);
     break;
   }
   LH_HIP(hipGetLastError());
@@ -1086,17 +1123,58 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
   return LH_OK;
 }
 
+int lhAcquireUpload(LhContext *c, float **hostBuffer)
+{
+  if (!c || !hostBuffer)
+    return fail(LH_E_ARG, "lhAcquireUpload: null argument");
+  if (c->ringAcquired >= 0)
+    return fail(LH_E_STATE, "lhAcquireUpload: previous buffer not committed");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  const int k = c->ringNext;
+  if (!c->hRing[k])
+    LH_HIP(hipHostMalloc((void **)&c->hRing[k], (uint64_t)c->cfg.W * c->cfg.H * sizeof(float),
+        hipHostMallocDefault));
+  // the buffer's previous H2D copy must have read it
+  LH_HIP(hipEventSynchronize(c->copyDone[k]));
+  c->ringAcquired = k;
+  *hostBuffer     = c->hRing[k];
+  return LH_OK;
+}
+
+int lhCommitUpload(LhContext *c, uint32_t slot, int id, const double *matrix)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhCommitUpload: null context");
+  const int k = c->ringAcquired;
+  if (k < 0)
+    return fail(LH_E_STATE, "lhCommitUpload: no buffer acquired");
+  c->ringAcquired = -1;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
+  // copy stream: wait until the staging kernel that last read dRing[k] ran
+  LH_HIP(hipStreamWaitEvent(c->upStream, c->stageDone[k], 0));
+  LH_HIP(hipMemcpyAsync(c->dRing[k], c->hRing[k], bytes, hipMemcpyHostToDevice, c->upStream));
+  LH_HIP(hipEventRecord(c->copyDone[k], c->upStream));
+  // compute stream: stage once the copy has landed
+  LH_HIP(hipStreamWaitEvent(c->stream, c->copyDone[k], 0));
+  const int rc = load_view_common(c, slot, id, matrix, c->dRing[k]);
+  if (rc != LH_OK)
+    return rc;
+  LH_HIP(hipEventRecord(c->stageDone[k], c->stream));
+  c->ringNext = (k + 1) % c->nRing;
+  return LH_OK;
+}
+
 int lhLoadView(LhContext *c, uint32_t slot, int id, const double *matrix, const float *image)
 {
   if (!c || !image)
     return fail(LH_E_ARG, "lhLoadView: null argument");
-  LH_HIP(hipSetDevice(c->cfg.device));
-  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
-  // the pinned staging buffer is reused: wait for the previous upload first
-  LH_HIP(hipStreamSynchronize(c->stream));
-  std::memcpy(c->hPinned, image, bytes);
-  LH_HIP(hipMemcpyAsync(c->dStage, c->hPinned, bytes, hipMemcpyHostToDevice, c->stream));
-  return load_view_common(c, slot, id, matrix);
+  float *buf = nullptr;
+  int rc     = lhAcquireUpload(c, &buf);
+  if (rc != LH_OK)
+    return rc;
+  std::memcpy(buf, image, (size_t)c->cfg.W * c->cfg.H * sizeof(float));
+  return lhCommitUpload(c, slot, id, matrix);
 }
 
 int lhLoadViewDevice(LhContext *c, uint32_t slot, int id, const double *matrix,
@@ -1105,34 +1183,51 @@ int lhLoadViewDevice(LhContext *c, uint32_t slot, int id, const double *matrix,
   if (!c || !dImage)
     return fail(LH_E_ARG, "lhLoadViewDevice: null argument");
   LH_HIP(hipSetDevice(c->cfg.device));
-  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
-  LH_HIP(hipMemcpyAsync(c->dStage, dImage, bytes, hipMemcpyDeviceToDevice, c->stream));
-  return load_view_common(c, slot, id, matrix);
+  return load_view_common(c, slot, id, matrix, dImage);
+}
+
+static void drop_graphs(LhContext *c)
+{
+  for (auto &kv : c->graphCache)
+    (void)hipGraphExecDestroy(kv.second);
+  c->graphCache.clear();
+}
+
+// Box table storage for nTiles tiles per slot (reallocation invalidates the
+// captured batches, which hold its address).
+static int box_storage(LhContext *c, long nTiles)
+{
+  if (c->boxTiles == nTiles)
+    return LH_OK;
+  drop_graphs(c);
+  (void)hipFree(c->dBoxes);
+  c->dBoxes   = nullptr;
+  c->boxTiles = 0;
+  LH_HIP(hipMalloc(&c->dBoxes, (size_t)c->cfg.nSlots * (size_t)nTiles * sizeof(int2)));
+  c->boxTiles = nTiles;
+  std::fill(c->boxR.begin(), c->boxR.end(), 0);
+  return LH_OK;
 }
 
 // Footprint boxes of slots [first, first+count) for the staged tile grid of
-// run length R; built once per (slot, R) and kept until the slot is reloaded.
+// run length R; built once per (slot, R) and kept until the slot is reloaded
+// (with `force`, rebuilt for every slot: the form a captured batch replays).
 static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, int tilesX,
-    int tilesY, int tilesZ)
+    int tilesY, int tilesZ, bool force)
 {
   const LhConfig &g = c->cfg;
   const long nTiles = (long)tilesX * tilesY * tilesZ;
-  if (c->boxTiles != nTiles) {
-    (void)hipFree(c->dBoxes);
-    c->dBoxes   = nullptr;
-    c->boxTiles = 0;
-    LH_HIP(hipMalloc(&c->dBoxes, (size_t)g.nSlots * (size_t)nTiles * sizeof(int2)));
-    c->boxTiles = nTiles;
-    std::fill(c->boxR.begin(), c->boxR.end(), 0);
-  }
+  const int rc      = box_storage(c, nTiles);
+  if (rc != LH_OK)
+    return rc;
   uint32_t s = first;
   while (s < first + count) {
-    if (c->boxR[s] == R) {
+    if (!force && c->boxR[s] == R) {
       ++s;
       continue;
     }
     uint32_t e = s;
-    while (e < first + count && c->boxR[e] != R && e - s < 65535)
+    while (e < first + count && (force || c->boxR[e] != R) && e - s < 65535)
       ++e;
     box_kernel<<<dim3((unsigned)((nTiles + 255) / 256), e - s), 256, 0, c->stream>>>(c->dViews,
         (int)s, (int)nTiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H, c->dBoxes);
@@ -1144,7 +1239,7 @@ static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, int
   return LH_OK;
 }
 
-static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
+static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBoxes = false)
 {
   const LhConfig &g = c->cfg;
   BpArgs A;
@@ -1172,7 +1267,7 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
   A.boxes       = nullptr;
   if (stage) {
-    const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, A.tilesZ);
+    const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, A.tilesZ, forceBoxes);
     if (rc != LH_OK)
       return rc;
     A.boxes = c->dBoxes;
@@ -1201,6 +1296,49 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count)
   return LH_OK;
 }
 
+// The batch [first, first+count) as one HIP graph per (slots, launch shape):
+// captured on first use (footprint boxes of every slot + the backprojection
+// launches), replayed afterwards; the slots' contents may change between
+// replays, their addresses do not.
+static int graph_backproject(LhContext *c, uint32_t first, uint32_t count, uint32_t per)
+{
+  const LhConfig &g = c->cfg;
+  const auto key = std::make_tuple(first, count, g.kernel * 4 + g.precision, (int)g.runLength,
+      g.kernel == LH_KERNEL_FUSED ? (int)(per << 1 | (uint32_t)g.staging) : g.staging);
+  auto it = c->graphCache.find(key);
+  if (it == c->graphCache.end()) {
+    if (g.staging) {
+      // allocate the box table outside the capture
+      const int R  = (int)g.runLength;
+      const int rc = box_storage(c, (long)(g.L / R) * (g.L / kLYs) * (g.L / kTileZ));
+      if (rc != LH_OK)
+        return rc;
+    }
+    hipGraph_t graph = nullptr;
+    LH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    int rc = LH_OK;
+    for (uint32_t s = first; s < first + count && rc == LH_OK; s += per)
+      rc = launch_bp(c, s, std::min(per, first + count - s), true);
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != LH_OK || e != hipSuccess) {
+      if (graph)
+        (void)hipGraphDestroy(graph);
+      return rc != LH_OK ? rc : fail(LH_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+      return fail(LH_E_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+    it = c->graphCache.emplace(key, exec).first;
+  }
+  LH_HIP(hipGraphLaunch(it->second, c->stream));
+  if (g.staging)
+    for (uint32_t s = first; s < first + count; ++s)
+      c->boxR[s] = (int)g.runLength;
+  return LH_OK;
+}
+
 int lhBackproject(LhContext *c, uint32_t first, uint32_t count)
 {
   if (!c)
@@ -1215,11 +1353,21 @@ int lhBackproject(LhContext *c, uint32_t first, uint32_t count)
       return fail(LH_E_STATE, "lhBackproject: slot %u holds no view", s);
   LH_HIP(hipSetDevice(g.device));
   const uint32_t per = g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 1u;
+  if (c->graphs)
+    return graph_backproject(c, first, count, per);
   for (uint32_t s = first; s < first + count; s += per) {
     const int rc = launch_bp(c, s, std::min(per, first + count - s));
     if (rc != LH_OK)
       return rc;
   }
+  return LH_OK;
+}
+
+int lhSetGraphs(LhContext *c, int enable)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhSetGraphs: null context");
+  c->graphs = enable != 0;
   return LH_OK;
 }
 

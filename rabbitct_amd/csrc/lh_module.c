@@ -252,6 +252,9 @@ int lolaHipPrepare(RabbitCtGlobalData *rcgd)
   int rc = lhCreate(&cfg, &Ctx);
   if (rc != LH_OK)
     die("device set-up", rc);
+  /* batches replay captured HIP graphs unless LOLAHIP_GRAPHS=0 */
+  if ((rc = lhSetGraphs(Ctx, env_int("LOLAHIP_GRAPHS", 1))) != LH_OK)
+    die("graph set-up", rc);
 
   LhLineRange *range = (LhLineRange *)malloc((size_t)N * L * L * sizeof(LhLineRange));
   if (range == NULL) {
@@ -276,6 +279,18 @@ int lolaHipPrepare(RabbitCtGlobalData *rcgd)
   return 1;
 }
 
+static void parallel_copy(float *dst, const float *src, size_t n)
+{
+  const long parts = 64;
+  const size_t chunk = (n + (size_t)parts - 1) / (size_t)parts;
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < parts; k++) {
+    const size_t a = (size_t)k * chunk;
+    if (a < n)
+      memcpy(dst + a, src + a, ((a + chunk < n) ? chunk : n - a) * sizeof(float));
+  }
+}
+
 static void flush_batch(void)
 {
   if (Pending == 0)
@@ -296,8 +311,14 @@ int lolaHipBackprojection(RabbitCtGlobalData *rcgd)
   int last = 0;
   for (uint32_t p = 0; p < rcgd->numberOfProjections; p++) {
     const Projection *pr = &rcgd->projectionBuffer[p];
-    int rc = lhLoadView(Ctx, BatchBase + Pending, pr->id, pr->matrix, pr->image);
+    /* the image goes through the context's pinned ring (copied by all host
+     * threads); its H2D copy and staging run behind the kernels in flight */
+    float *buf = NULL;
+    int rc     = lhAcquireUpload(Ctx, &buf);
     if (rc != LH_OK)
+      die("view upload", rc);
+    parallel_copy(buf, pr->image, (size_t)rcgd->imageWidth * rcgd->imageHeight);
+    if ((rc = lhCommitUpload(Ctx, BatchBase + Pending, pr->id, pr->matrix)) != LH_OK)
       die("view upload", rc);
     Pending++;
     if ((uint32_t)pr->id == TotalViews)

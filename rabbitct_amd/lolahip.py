@@ -74,6 +74,7 @@ EXPORTS = (
     "lhLoadView", "lhLoadViewDevice", "lhBackproject", "lhTimedBackproject",
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
     "lhSetLaunch", "lhCopyCalibration", "lhVolumeDevice", "lhScaleVolume",
+    "lhAcquireUpload", "lhCommitUpload", "lhSetGraphs",
     # sparse-view + learned correction
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
@@ -132,6 +133,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhAbiLayout.restype = None
     lib.lhVolumeDevice.argtypes = [P, ctypes.POINTER(P)]
     lib.lhScaleVolume.argtypes = [P, f32]
+    lib.lhAcquireUpload.argtypes = [P, ctypes.POINTER(P)]
+    lib.lhCommitUpload.argtypes = [P, u32, i32, P]
+    lib.lhSetGraphs.argtypes = [P, i32]
     u64, f64 = ctypes.c_uint64, ctypes.c_double
     lib.lhLineIntegrals.argtypes = [i32, i32, i32, u32, P, P, u32, P, P]
     lib.lhForwardProject.argtypes = [P, i32, f32, f32, i32, i32, i32, u32, P, P, f32, P]
@@ -253,6 +257,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_graphs(self, enable: bool) -> None:
+        _check(self.lib.lhSetGraphs(self._h, 1 if enable else 0), "lhSetGraphs")
 
     def scale_volume(self, s: float) -> None:
         _check(self.lib.lhScaleVolume(self._h, s), "lhScaleVolume")

@@ -255,3 +255,17 @@ def test_ramp_filter_matches_fft_convolution():
     n = 256
     Y = np.fft.irfft(np.fft.rfft(x, n) * np.fft.rfft(np.roll(np.pad(h, (0, n - h.size)), -63), n), n)
     np.testing.assert_allclose(y, tau * Y[:, :64], atol=1e-9)
+
+
+def test_backproject_lines_equals_clipped_volume(golden_border):
+    """The line-subset form is the clipped loop on those lines."""
+    g = golden_border
+    L = 128
+    rng = np.random.default_rng(5)
+    zs = rng.integers(0, L, 40).astype(np.int32)
+    ys = rng.integers(0, L, 40).astype(np.int32)
+    full = orc.backproject(L, voxel_size(L), volume_origin(L), g["matrices"], g["images"],
+                           g["clip16"])
+    lines = orc.backproject_lines(L, voxel_size(L), volume_origin(L), g["matrices"],
+                                  g["images"], g["clip16"], zs, ys)
+    np.testing.assert_array_equal(lines, full[zs, ys, :])
