@@ -305,3 +305,43 @@ def load_weights(net: unet.UNet, path: str) -> dict:
             raise ValueError(f"{path}: layer table does not match")
         net.set_params(z["params"])
         return json.loads(str(z["meta"])) if "meta" in z else {}
+
+
+def main(argv=None) -> int:
+    """Timing entry:  python -m rabbitct_amd.learned {unet,pipeline} [--L 256]
+    [--reps 6] -- wall time per U-Net forward (random input, committed
+    weights) or per sparse-view reconstruction of a scan outside the training
+    set, each synchronised."""
+    import argparse
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("what", choices=["unet", "pipeline"])
+    ap.add_argument("--L", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=6)
+    args = ap.parse_args(argv)
+    if args.what == "unet":
+        net = unet.UNet(seed=0)
+        load_weights(net, DEFAULT_WEIGHTS)
+        v = np.random.default_rng(0).normal(size=(1,) + (args.L,) * 3).astype(np.float32)
+        with unet.DeviceBuffer.from_array(v) as din, unet.DeviceBuffer(v.nbytes) as dout:
+            for i in range(args.reps):
+                t = time.perf_counter()
+                net.forward(din, dout, 1, args.L)
+                print(f"forward {i}: {(time.perf_counter() - t) * 1e3:.2f} ms", flush=True)
+        net.close()
+    else:
+        scan = make_scan(args.L, 200000)
+        rec = SparseViewReconstructor()
+        for i in range(args.reps):
+            r = rec.reconstruct(scan)
+            print(f"reconstruct {i}: " + " ".join(f"{k} {v:.2f} ms" for k, v in r["ms"].items()) +
+                  f" fallback {r['fallback']} residual {r['residual']:.4f}", flush=True)
+        t = time.perf_counter()
+        scan.full_view(*rec.full_kernel)
+        print(f"full view plain: {(time.perf_counter() - t) * 1e3:.2f} ms", flush=True)
+        scan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
