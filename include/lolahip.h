@@ -181,6 +181,8 @@ int lhLoadViewDevice(LhContext *ctx, uint32_t slot, int id, const double *matrix
  * staging into `slot` on the compute stream, ordered by events. */
 int lhAcquireUpload(LhContext *ctx, float **hostBuffer);
 int lhCommitUpload(LhContext *ctx, uint32_t slot, int id, const double *matrix);
+/* Gives an acquired buffer back without uploading it. */
+int lhCancelUpload(LhContext *ctx);
 
 /* Enqueues the backprojection of slots [firstSlot, firstSlot+count) on the
  * context's stream (asynchronous). */
@@ -297,6 +299,39 @@ int lhDeviceAlloc(void **p, uint64_t bytes);
 int lhDeviceFree(void *p);
 int lhCopyToDevice(void *dDst, const void *src, uint64_t bytes);
 int lhCopyToHost(void *dst, const void *dSrc, uint64_t bytes);
+
+/* ---------------------------------------------------------------------
+ * Before and after the path.
+ * ------------------------------------------------------------------- */
+
+/* The context's configuration (as created, with later lhSetLaunch changes). */
+int lhGetConfig(const LhContext *ctx, LhConfig *out);
+
+/* Reads views [firstView, firstView+count) of a .rct file (the record layout
+ * of ctFileReader.c:51-58) straight into slots firstSlot.. (ids firstView+1..)
+ * through the pinned upload ring, with all host threads issuing pread, so
+ * reads overlap the copies and staging of earlier views; matrices (optional,
+ * count x 12) receives their matrices. */
+int lhLoadRctFile(LhContext *ctx, const char *path, uint32_t firstView, uint32_t count,
+    uint32_t firstSlot, double *matrices);
+
+/* FDK pre-processing of measured line integrals (count x H x W floats,
+ * device) into the filtered projections the RabbitCT input carries: cosine
+ * weight from the ray direction (minv, count x 9) against the principal axis
+ * (axis, count x 3), column weights parker (count x W, e.g. short-scan
+ * redundancy), row ramp filter of sample spacing tau (mm), times dbeta (rad),
+ * optional zero 1-pixel border.  dOut: count x H x W floats, device. */
+int lhFdkFilter(int W, int H, uint32_t count, const double *minv, const double *axis,
+    const float *parker, double tau, double dbeta, int border, const float *dRaw, float *dOut);
+
+/* The runner's -c verification (rabbitRunner.c:346-384) on the GPU:
+ * HU = floor(hus0*v + hus1 + 0.5) for v > 0 (else 0), clamped to 4095, against
+ * a uint16 reference (device); mse, max |error| and the 11-bin histogram of
+ * floor|error| (last bin >= 10).  Exact (integer sums). */
+int lhHuMetric(const float *dVol, uint64_t n, const uint16_t *dRef, float hus0, float hus1,
+    double *mse, double *maxAbs, uint32_t *hist);
+/* The same HU conversion into a uint16 volume (device). */
+int lhHuConvert(const float *dVol, uint64_t n, float hus0, float hus1, uint16_t *dOut);
 
 /* Last error message of the calling thread. */
 const char *lhLastError(void);

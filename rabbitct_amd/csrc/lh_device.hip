@@ -1069,6 +1069,14 @@ int lhScaleVolume(LhContext *c, float s)
   return LH_OK;
 }
 
+int lhGetConfig(const LhContext *c, LhConfig *out)
+{
+  if (!c || !out)
+    return fail(LH_E_ARG, "lhGetConfig: null argument");
+  *out = c->cfg;
+  return LH_OK;
+}
+
 int lhVolumeDevice(LhContext *c, float **dVol)
 {
   if (!c || !dVol)
@@ -1138,6 +1146,14 @@ int lhAcquireUpload(LhContext *c, float **hostBuffer)
   LH_HIP(hipEventSynchronize(c->copyDone[k]));
   c->ringAcquired = k;
   *hostBuffer     = c->hRing[k];
+  return LH_OK;
+}
+
+int lhCancelUpload(LhContext *c)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhCancelUpload: null context");
+  c->ringAcquired = -1;  // the buffer stays next in the ring
   return LH_OK;
 }
 

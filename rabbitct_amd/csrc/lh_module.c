@@ -16,10 +16,12 @@
  */
 #define _GNU_SOURCE
 #include <errno.h>
+#include <fcntl.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "lolahip.h"
 
@@ -345,4 +347,88 @@ int lolaHipFinish(RabbitCtGlobalData *rcgd)
   lhDestroy(Ctx);
   Ctx = NULL;
   return 1;
+}
+
+/* ---------------------------------------------------------------------------
+ * .rct ingest straight into the device (no counterpart in the reference, whose
+ * runner reads each record into its own buffers, ctFileReader.c:47-61): views
+ * [firstView, firstView+count) are read with pread by all host threads into
+ * the context's pinned upload ring and committed to slots firstSlot.., so the
+ * file reads overlap the H2D copies and staging of earlier views.  The
+ * record layout is ctFileReader.c:51-58.  matrices (optional, count x 12)
+ * receives the views' matrices.
+ * ------------------------------------------------------------------------- */
+static int pread_all(int fd, char *dst, size_t len, off_t off)
+{
+  while (len > 0) {
+    const ssize_t r = pread(fd, dst, len, off);
+    if (r <= 0) {
+      if (r < 0 && errno == EINTR)
+        continue;
+      return -1;
+    }
+    dst += r;
+    len -= (size_t)r;
+    off += r;
+  }
+  return 0;
+}
+
+int lhLoadRctFile(LhContext *ctx, const char *path, uint32_t firstView, uint32_t count,
+    uint32_t firstSlot, double *matrices)
+{
+  LhConfig cfg;
+  if (!ctx || !path || lhGetConfig(ctx, &cfg) != LH_OK)
+    return LH_E_ARG;
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) {
+    fprintf(stderr, "LolaHIP: cannot open %s: %s\n", path, strerror(errno));
+    return LH_E_IO;
+  }
+  uint32_t hdr[5];
+  int rc = LH_OK;
+  if (pread_all(fd, (char *)hdr, sizeof hdr, 0) != 0) {
+    rc = LH_E_IO;
+  } else if (hdr[0] != cfg.W || hdr[1] != cfg.H || firstView + count > hdr[2]) {
+    fprintf(stderr, "LolaHIP: %s holds %u views of %ux%u, asked for views [%u,%u) of %ux%u\n",
+        path, hdr[2], hdr[0], hdr[1], firstView, firstView + count, cfg.W, cfg.H);
+    rc = LH_E_ARG;
+  }
+  const size_t imgBytes = (size_t)cfg.W * cfg.H * sizeof(float);
+  const size_t record   = 12 * sizeof(double) + imgBytes;
+  for (uint32_t v = 0; rc == LH_OK && v < count; v++) {
+    const off_t off = (off_t)20 + (off_t)(firstView + v) * (off_t)record;
+    double mat[12];
+    float *buf = NULL;
+    if (pread_all(fd, (char *)mat, sizeof mat, off) != 0) {
+      rc = LH_E_IO;
+      break;
+    }
+    if ((rc = lhAcquireUpload(ctx, &buf)) != LH_OK)
+      break;
+    const long parts   = 16;
+    const size_t chunk = (imgBytes + (size_t)parts - 1) / (size_t)parts;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (long k = 0; k < parts; k++) {
+      const size_t a = (size_t)k * chunk;
+      if (a < imgBytes) {
+        const size_t len = (a + chunk < imgBytes) ? chunk : imgBytes - a;
+        bad |= pread_all(fd, (char *)buf + a, len, off + (off_t)sizeof mat + (off_t)a) != 0;
+      }
+    }
+    if (bad) {
+      (void)lhCancelUpload(ctx);
+      rc = LH_E_IO;
+      break;
+    }
+    if ((rc = lhCommitUpload(ctx, firstSlot + v, (int)(firstView + v + 1), mat)) != LH_OK)
+      break;
+    if (matrices)
+      memcpy(matrices + 12 * (size_t)v, mat, sizeof mat);
+  }
+  close(fd);
+  if (rc == LH_E_IO)
+    fprintf(stderr, "LolaHIP: short read from %s\n", path);
+  return rc;
 }

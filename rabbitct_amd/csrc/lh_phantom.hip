@@ -119,6 +119,24 @@ __global__ void raw_integrals_kernel(int Wd, int stride, const double *__restric
   out[(size_t)j * Wd + i] = (float)acc;
 }
 
+// FDK weights of measured line integrals: out = raw * cos(angle to the
+// principal axis) * parker[column], FP64 (the form integrals_kernel produces).
+__global__ void fdk_weight_kernel(int W, const double *__restrict__ minv,
+    const double *__restrict__ axis, const float *__restrict__ parker,
+    const float *__restrict__ raw, double *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= W)
+    return;
+  double d[3];
+  for (int r = 0; r < 3; ++r)
+    d[r] = minv[3 * r + 0] * (double)i + minv[3 * r + 1] * (double)j + minv[3 * r + 2];
+  const double nrm  = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double cosw = (d[0] * axis[0] + d[1] * axis[1] + d[2] * axis[2]) / nrm;
+  out[(size_t)j * W + i] = (double)raw[(size_t)j * W + i] * cosw * (double)parker[i];
+}
+
 } // namespace
 
 // Unweighted line integrals of an ellipsoid phantom on a detector subsampled
@@ -160,6 +178,51 @@ extern "C" int lhLineIntegrals(int W, int H, int stride, uint32_t count, const d
   (void)hipFree(dGeo);
   (void)hipFree(dEll);
   return rc;
+}
+
+// FDK pre-processing of measured projections (the step before the
+// backprojection; the RabbitCT input carries its result): dRaw (count x H x W
+// line integrals, device) -> dOut (device) = ramp_tau( raw * cos * parker ) *
+// dbeta, optionally with a zero 1-pixel border.  minv / axis / parker as in
+// lhGenerateViews (host).
+extern "C" int lhFdkFilter(int W, int H, uint32_t count, const double *minv, const double *axis,
+    const float *parker, double tau, double dbeta, int border, const float *dRaw, float *dOut)
+{
+  if (W < 2 || H < 1 || !minv || !axis || !parker || !dRaw || !dOut || !(tau > 0.0))
+    return lh_fail(LH_E_ARG, "lhFdkFilter: bad arguments");
+  double *dGeo = nullptr, *dW = nullptr;
+  float *dPar = nullptr;
+  std::vector<double> geo((size_t)count * 12);
+  for (uint32_t v = 0; v < count; ++v) {
+    for (int k = 0; k < 9; ++k)
+      geo[12 * v + k] = minv[9 * v + k];
+    for (int k = 0; k < 3; ++k)
+      geo[12 * v + 9 + k] = axis[3 * v + k];
+  }
+  hipError_t e = hipMalloc(&dGeo, geo.size() * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&dW, (size_t)W * H * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&dPar, (size_t)count * W * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMemcpy(dGeo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dPar, parker, (size_t)count * W * sizeof(float), hipMemcpyHostToDevice);
+  for (uint32_t v = 0; e == hipSuccess && v < count; ++v) {
+    const size_t img = (size_t)v * W * H;
+    fdk_weight_kernel<<<dim3((W + 255) / 256, H), 256>>>(W, dGeo + 12 * v, dGeo + 12 * v + 9,
+        dPar + (size_t)v * W, dRaw + img, dW);
+    ramp_kernel<<<H, 256, W * sizeof(double)>>>(W, H, tau, dbeta, border, dW, dOut + img);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipDeviceSynchronize();
+  (void)hipFree(dGeo);
+  (void)hipFree(dW);
+  (void)hipFree(dPar);
+  if (e != hipSuccess)
+    return lh_fail(LH_E_HIP, "lhFdkFilter: %s", hipGetErrorString(e));
+  return LH_OK;
 }
 
 extern "C" int lhGenerateViews(LhContext *c, uint32_t firstSlot, uint32_t count, const int *ids,

@@ -74,12 +74,14 @@ EXPORTS = (
     "lhLoadView", "lhLoadViewDevice", "lhBackproject", "lhTimedBackproject",
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
     "lhSetLaunch", "lhCopyCalibration", "lhVolumeDevice", "lhScaleVolume",
-    "lhAcquireUpload", "lhCommitUpload", "lhSetGraphs",
+    "lhAcquireUpload", "lhCommitUpload", "lhCancelUpload", "lhSetGraphs",
     # sparse-view + learned correction
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
     "lhUnetGetGrads", "lhUnetForward", "lhUnetTrainStep", "lhConv3dTest", "lhConv3dWgradTest",
     "lhDeviceAlloc", "lhDeviceFree", "lhCopyToDevice", "lhCopyToHost",
+    # before and after the path
+    "lhFdkFilter", "lhHuMetric", "lhHuConvert", "lhGetConfig", "lhLoadRctFile",
 )
 
 _lib = None
@@ -135,6 +137,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhScaleVolume.argtypes = [P, f32]
     lib.lhAcquireUpload.argtypes = [P, ctypes.POINTER(P)]
     lib.lhCommitUpload.argtypes = [P, u32, i32, P]
+    lib.lhCancelUpload.argtypes = [P]
     lib.lhSetGraphs.argtypes = [P, i32]
     u64, f64 = ctypes.c_uint64, ctypes.c_double
     lib.lhLineIntegrals.argtypes = [i32, i32, i32, u32, P, P, u32, P, P]
@@ -156,6 +159,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhDeviceFree.argtypes = [P]
     lib.lhCopyToDevice.argtypes = [P, P, u64]
     lib.lhCopyToHost.argtypes = [P, P, u64]
+    lib.lhFdkFilter.argtypes = [i32, i32, u32, P, P, P, f64, f64, i32, P, P]
+    lib.lhHuMetric.argtypes = [P, u64, P, f32, f32, ctypes.POINTER(f64), ctypes.POINTER(f64), P]
+    lib.lhHuConvert.argtypes = [P, u64, f32, f32, P]
+    lib.lhGetConfig.argtypes = [P, ctypes.POINTER(LhConfig)]
+    lib.lhLoadRctFile.argtypes = [P, ctypes.c_char_p, u32, u32, u32, P]
     _lib = lib
     return lib
 
@@ -257,6 +265,14 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def load_rct(self, path, first_view: int, count: int, first_slot: int = 0):
+        """Views [first_view, first_view+count) of a .rct file into slots
+        first_slot..; returns their (count, 12) matrices."""
+        m = np.zeros((count, 12), np.float64)
+        _check(self.lib.lhLoadRctFile(self._h, os.fsencode(str(path)), first_view, count,
+                                      first_slot, _p(m)), "lhLoadRctFile")
+        return m
 
     def set_graphs(self, enable: bool) -> None:
         _check(self.lib.lhSetGraphs(self._h, 1 if enable else 0), "lhSetGraphs")

@@ -277,3 +277,29 @@ def test_l1024_sampled_lines_vs_fp64(kernel, prec):
 def generate_into_host(ctx, geom, ph):
     from rabbitct_amd.generate import generate_into
     return generate_into(ctx, geom, ph, want_host=True)
+
+
+def test_rct_file_ingest_equals_host_upload(tmp_path, golden_border):
+    """lhLoadRctFile (parallel pread into the pinned ring) stages the same
+    slots as lhLoadView: identical volumes; the matrices come back."""
+    g = golden_border
+    N, H, W = g["images"].shape
+    p = tmp_path / "in.rct"
+    rct.write_rct(p, rct.RctHeader(W, H, N, (1000.0, 0.0)), g["matrices"], g["images"])
+    vols = []
+    for via_file in (False, True):
+        with lolahip.Context(L, W, H, N, n_slots=N) as ctx:
+            ctx.set_ranges(g["clip16"])
+            if via_file:
+                m = ctx.load_rct(p, 2, N - 2, first_slot=2)
+                assert np.array_equal(m, g["matrices"][2:])
+                ctx.load_rct(p, 0, 2, first_slot=0)
+            else:
+                for v in range(N):
+                    ctx.load_view(v, v + 1, g["matrices"][v], g["images"][v])
+            ctx.backproject(0, N)
+            vols.append(ctx.get_volume())
+        with lolahip.Context(L, W, H, N, n_slots=N) as ctx:
+            with pytest.raises(lolahip.LolaHipError):
+                ctx.load_rct(p, 5, N, first_slot=0)          # past the end of the file
+    assert np.array_equal(vols[0], vols[1])

@@ -1,0 +1,62 @@
+"""GPU forms of the steps either side of the path: the -c metric against the
+numbers the compiled runner printed, and FDK pre-processing against the
+generator and the FP64 oracle chain."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from rabbitct_amd import lolahip, prepost, unet
+from rabbitct_amd.generate import generate_into
+from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry
+
+pytestmark = pytest.mark.gpu
+
+L = 128
+
+
+def test_hu_metric_matches_runner(golden):
+    rng = np.random.default_rng(0)
+    vol = rng.uniform(-0.5, 3.0, L ** 3).astype(np.float32)
+    h0, h1 = golden["hu"]
+    ref = orc.hu_convert(vol, h0, h1)
+    ref[golden["pert_idx"]] += golden["pert_val"]
+    with unet.DeviceBuffer.from_array(vol) as dv, unet.DeviceBuffer.from_array(ref) as dr:
+        m = prepost.hu_metric(dv, dr, vol.size, h0, h1)
+    o = orc.hu_metric(vol, ref, h0, h1)
+    assert m["mse"] == o["mse"] and m["max"] == o["max"]
+    assert np.array_equal(m["hist"], o["hist"])
+    assert m["mse"] == pytest.approx(float(golden["check_mse"]), rel=1e-5)
+    assert m["max"] == float(golden["check_max"])
+    assert m["psnr"] == pytest.approx(float(golden["check_psnr"]), rel=1e-5)
+    assert np.array_equal(m["hist"], golden["check_hist"])
+
+
+def test_hu_convert_matches_runner_pgm(golden):
+    h0, h1 = golden["hu"]
+    s = np.ascontiguousarray(golden["opt16_slice"], dtype=np.float32)
+    with unet.DeviceBuffer.from_array(s) as dv, unet.DeviceBuffer(s.size * 2) as do:
+        prepost.hu_convert(dv, s.size, h0, h1, do)
+        got = do.download(s.shape, np.uint16)
+    assert np.array_equal(got, golden["pgm_slice"])
+
+
+def test_fdk_filter_matches_generator_and_oracle():
+    geom = ScanGeometry(n_views=496, seed=77).build()
+    views = np.array([0, 100, 333])
+    ph = EllipsoidPhantom(seed=77).build()
+    W, H = geom.width, geom.height
+    S, Minv, _ = geom.rays()
+    with lolahip.Context(128, W, H, geom.n_views, n_slots=3) as ctx:
+        gen = generate_into(ctx, geom, ph, views=views, want_host=True)
+    with unet.DeviceBuffer(3 * W * H * 4) as raw, unet.DeviceBuffer(3 * W * H * 4) as out:
+        unet.line_integrals(W, H, 1, S[views], Minv[views], ph.params, raw)
+        prepost.fdk_filter(geom, raw, out, views=views)
+        got = out.download((3, H, W))
+    scale = np.abs(gen).max()
+    # the raw integrals pass through float32 here, the generator keeps FP64
+    assert np.abs(got - gen).max() <= 2e-5 * scale
+    ref = orc.fdk_projections(geom.subset(views), ph)       # FP64 chain
+    # subset() keeps the angles; the oracle's angular step uses the subset's
+    # view count, the filter the scan's
+    ref *= len(views) / geom.n_views
+    assert np.abs(got - ref).max() <= 2e-5 * scale
