@@ -273,8 +273,18 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
 #ifndef LH_FENCE
 #define LH_FENCE 4 // updates per scheduling group in the staged loop
 #endif
-constexpr int kFW    = 128; // LDS row pitch (floats); ds_read2 offsets stay <= 255
-constexpr int kFH    = 39;  // LDS rows per buffer (2 buffers x 39 x 128 x 4 B: 4 workgroups per CU)
+#ifndef LH_PITCH
+#define LH_PITCH 128
+#endif
+#ifndef LH_FH
+#define LH_FH 39
+#endif
+constexpr int kFW    = 128;      // widest footprint staged (pixels)
+constexpr int kPitch = LH_PITCH; // LDS row pitch (dwords) >= kFW; ds_read2 offsets stay <= 255
+constexpr int kFH    = LH_FH;    // LDS rows per buffer (2 buffers x kFH x kPitch x 4 B <= 40 KB:
+                                 // 4 workgroups per CU)
+static_assert(kPitch >= kFW && kPitch + 1 <= 255, "LDS pitch");
+static_assert(2 * kFH * kPitch * 4 <= 40 * 1024, "footprint buffers exceed 40 KB");
 constexpr int kLYs   = 32;  // lanes along y in the staged tile
 constexpr int kTileZ = 8;   // z extent of the staged tile
 
@@ -352,14 +362,15 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
   boxes[(uint64_t)slot * (uint64_t)nTiles + t] = q;
 }
 
-// Issues the direct-to-LDS loads of footprint `b` of the slot image into buf:
-// wave w copies rows w, w+4, ...; each instruction moves pixels
-// [iu0 + c0, iu0 + c0 + 64) of one row into buf[r*kFW + c0 ..].  Columns at or
-// beyond fw are copied too and never read; past the end of the slot the
-// buffer bounds return zero.  Completion is awaited by the caller
-// (s_waitcnt vmcnt(0) and a barrier).
-// elem = bytes per slot element: 4 (FP32 image) or 8 (half "quad" image, of
-// which the first dword, the horizontal pair p(i,j), p(i+1,j), is copied).
+// Issues the direct-to-LDS loads of footprint `b` of the slot image into buf.
+// FP32: each buffer_load_dwordx4 ... lds moves two rows of 128 pixels (lane l
+// lands 16 bytes at l*16, so lanes 32-63 fill the next LDS row); wave w copies
+// row pairs 2w, 2w+8, ....  Halves (elem 8, the "quad" image of which the
+// first dword, the horizontal pair p(i,j), p(i+1,j), is copied): one dword per
+// lane, pixels [iu0 + c0, iu0 + c0 + 64) of a row per instruction, wave w
+// copying rows w, w+4, ....  Columns at or beyond fw are copied too and never
+// read; past the end of the slot the buffer bounds return zero.  Completion is
+// awaited by the caller (s_waitcnt vmcnt(0) and a barrier).
 template <int ELEM>
 __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StageBox &b,
     int pitch, float *buf)
@@ -367,12 +378,26 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = threadIdx.x / kWave;
   const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * ELEM;
-  for (int r = wv; r < b.fh; r += kWaves) {
-    const uint32_t row = base + (uint32_t)(r * pitch) * ELEM;
-    for (int c0 = 0; c0 < b.fw; c0 += kWave) {
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-          (__attribute__((address_space(3))) void *)(buf + r * kFW + c0), 4,
-          (int)(row + (uint32_t)(c0 + lane) * ELEM), 0, 0, 0);
+  if constexpr (ELEM == 4 && kPitch == 128) {
+    // 16 bytes (4 pixels) per lane: lanes 0-31 stage row r, lanes 32-63 row
+    // r+1, which lands 512 bytes (one LDS row) further; rows past fh are
+    // masked off.  Columns past fw (up to 128) are staged and never read.
+    const int half = lane >> 5, col = (lane & 31) * 4;
+    for (int r0 = 2 * wv; r0 < b.fh; r0 += 2 * kWaves) {
+      const int r = r0 + half;
+      if (r < b.fh)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(buf + r0 * kPitch), 16,
+            (int)(base + (uint32_t)(r * pitch + col) * 4u), 0, 0, 0);
+    }
+  } else {
+    for (int r = wv; r < b.fh; r += kWaves) {
+      const uint32_t row = base + (uint32_t)(r * pitch) * ELEM;
+      for (int c0 = 0; c0 < b.fw; c0 += kWave) {
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(buf + r * kPitch + c0), 4,
+            (int)(row + (uint32_t)(c0 + lane) * ELEM), 0, 0, 0);
+      }
     }
   }
 }
@@ -449,7 +474,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     if (hasNext) {
       boxNext = load_box(A, slot + 1, tile);
       if (blockActNext && boxNext.fits)
-        stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kFW * kFH);
+        stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kPitch * kFH);
     }
 
     const int start   = (int)(short)(rr & 0xffff);
@@ -464,8 +489,8 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       const float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
       const float du = P.du, dv = P.dv, dw = P.dw;
       if (box.fits) {
-        // sBase[v*kFW + u] is footprint pixel (u, v): the window origin folded in
-        const float *sBase = sImg + (v & 1) * kFW * kFH - (box.iv0 * kFW + box.iu0);
+        // sBase[v*kPitch + u] is footprint pixel (u, v): the window origin folded in
+        const float *sBase = sImg + (v & 1) * kPitch * kFH - (box.iv0 * kPitch + box.iu0);
         auto update = [&](int r) -> float {
           const float fr = (float)r;
           const float rw = __builtin_amdgcn_rcpf(fmaf(fr, dw, w0));
@@ -473,20 +498,20 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
           const float iy = __builtin_amdgcn_fmed3f(fmaf(fr, dv, v0) * rw, box.lov, box.hiv);
           const float fx = __builtin_truncf(ix);
           const float fy = __builtin_truncf(iy);
-          // fy*kFW + fx is an exact integer in FP32 (|.| < 2^24): one conversion
-          const int a    = (int)fmaf(fy, (float)kFW, fx);
+          // fy*kPitch + fx is an exact integer in FP32 (|.| < 2^24): one conversion
+          const int a    = (int)fmaf(fy, (float)kPitch, fx);
           const float ax = ix - fx, ay = iy - fy;
           float b, t;
           if constexpr (PREC == LH_PREC_F32) {
             const float *q  = sBase + a;
             const float p00 = q[0], p10 = q[1];
-            const float p01 = q[kFW], p11 = q[kFW + 1];
+            const float p01 = q[kPitch], p11 = q[kPitch + 1];
             b = fmaf(ax, p10 - p00, p00);
             t = fmaf(ax, p11 - p01, p01);
           } else {
             // one dword per pixel: the horizontal pair (p(i,j), p(i+1,j))
             const uint32_t *q = reinterpret_cast<const uint32_t *>(sBase) + a;
-            const uint32_t q0 = q[0], q1 = q[kFW];
+            const uint32_t q0 = q[0], q1 = q[kPitch];
             if constexpr (PREC == LH_PREC_F16) {
               // (1-ax) p0 + ax p1 with the FP16 operands widened inside the FMA
               b = fmaf(ax, f16_hi(q0), fmaf(-ax, f16_lo(q0), f16_lo(q0)));
@@ -557,7 +582,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 template <int R, int PREC, int STAGE>
 __global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpArgs A)
 {
-  __shared__ float sImg[STAGE ? 2 * kFW * kFH : 1];
+  __shared__ float sImg[STAGE ? 2 * kPitch * kFH : 1];
   __shared__ int sAct[2 * kWaves];
   if (!A.persistent) {
     if constexpr (STAGE)
