@@ -133,9 +133,9 @@ __device__ __forceinline__ float f16_hi(uint32_t q)
 
 // The four taps of the bilinear stencil at integer corner (iix, iiy), both
 // already clamped to [-2, W] x [-2, H].
-//  FP32: row-major padded image, two 8-byte loads (rows iiy and iiy+1).
-//  BF16/F16: "quad" layout, element (i,j) holds p(i,j), p(i+1,j), p(i,j+1),
-//  p(i+1,j+1) as 4 halves: one 8-byte load.
+//  FP32: row-major padded image, four 4-byte loads (rows iiy and iiy+1).
+//  BF16/F16: "pair" layout, element (i,j) holds p(i,j), p(i+1,j) as 2 halves
+//  in one dword: two 4-byte loads (rows iiy and iiy+1).
 template <int PREC>
 __device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, const BpArgs &A,
     int iix, int iiy, float &p00, float &p10, float &p01, float &p11)
@@ -148,10 +148,10 @@ __device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, const BpAr
     p01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0));
     p11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, (int)A.rowBytes, 0));
   } else {
-    const int pitch    = (int)(A.rowBytes >> 3);
-    const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 8u + A.baseOffset;
+    const int pitch    = (int)(A.rowBytes >> 2);
+    const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
     const uint32_t q0 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
-    const uint32_t q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0);
+    const uint32_t q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0);
     if constexpr (PREC == LH_PREC_BF16) {
       p00 = bf16_lo(q0); p10 = bf16_hi(q0); p01 = bf16_lo(q1); p11 = bf16_hi(q1);
     } else {
@@ -365,10 +365,10 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
 // Issues the direct-to-LDS loads of footprint `b` of the slot image into buf.
 // FP32: each buffer_load_dwordx4 ... lds moves two rows of 128 pixels (lane l
 // lands 16 bytes at l*16, so lanes 32-63 fill the next LDS row); wave w copies
-// row pairs 2w, 2w+8, ....  Halves (elem 8, the "quad" image of which the
-// first dword, the horizontal pair p(i,j), p(i+1,j), is copied): one dword per
-// lane, pixels [iu0 + c0, iu0 + c0 + 64) of a row per instruction, wave w
-// copying rows w, w+4, ....  Columns at or beyond fw are copied too and never
+// row pairs 2w, 2w+8, ... (a half image holds one dword per element too: the
+// horizontal pair p(i,j), p(i+1,j)).  Other layouts: one dword per lane,
+// pixels [iu0 + c0, iu0 + c0 + 64) of a row per instruction, wave w copying
+// rows w, w+4, ....  Columns at or beyond fw are copied too and never
 // read; past the end of the slot the buffer bounds return zero.  Completion is
 // awaited by the caller (s_waitcnt vmcnt(0) and a barrier).
 template <int ELEM>
@@ -406,7 +406,7 @@ template <int R, int PREC>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
 {
-  constexpr int ELEM = PREC == LH_PREC_F32 ? 4 : 8;
+  constexpr int ELEM = 4;  // FP32 pixel or half pair: one dword per element
   const int tile = tx + A.tilesX * (ty + A.tilesY * tz);
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = threadIdx.x / kWave;
@@ -656,8 +656,8 @@ __device__ __forceinline__ uint16_t to_f16(float f)
 }
 
 // Writes the bordered slot image from a W x H float image.  FP32: element
-// (i,j), i in [-2, W+1], j in [-2, H+1]; halves: quad element (i,j), i in
-// [-2, W], j in [-2, H].  Block (0,0) thread 0 also applies the storage scale
+// (i,j), i in [-2, W+1], j in [-2, H+1]; halves: pair element (i,j), i in
+// [-2, W], j in [-2, H+1].  Block (0,0) thread 0 also applies the storage scale
 // to the slot's view parameters.
 template <int PREC>
 __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
@@ -690,15 +690,12 @@ __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_
   } else {
     if (ei >= W + 3)
       return;
-    uint16_t h[4];
-    const float v[4] = { pix(i, j) * is, pix(i + 1, j) * is, pix(i, j + 1) * is,
-      pix(i + 1, j + 1) * is };
-    for (int c = 0; c < 4; ++c)
+    uint16_t h[2];
+    const float v[2] = { pix(i, j) * is, pix(i + 1, j) * is };
+    for (int c = 0; c < 2; ++c)
       h[c] = PREC == LH_PREC_BF16 ? to_bf16(v[c]) : to_f16(v[c]);
-    uint2 q;
-    q.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-    q.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-    reinterpret_cast<uint2 *>(slot + (size_t)ej * rowBytes)[ei] = q;
+    const uint32_t q = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    reinterpret_cast<uint32_t *>(slot + (size_t)ej * rowBytes)[ei] = q;
   }
 }
 
@@ -903,11 +900,11 @@ int lhCreate(const LhConfig *cfg, LhContext **out)
   LH_HIP(hipMalloc(&c->dViews, cfg->nSlots * sizeof(ViewParams)));
   LH_HIP(hipMemsetAsync(c->dViews, 0, cfg->nSlots * sizeof(ViewParams), c->stream));
 
-  // slot layout: FP32 rows of W+4 floats, H+4 rows; halves: rows of W+3 quads
-  // (8 bytes), H+3 rows.  Rows padded to 256 bytes.
-  const uint64_t elem  = cfg->precision == LH_PREC_F32 ? 4 : 8;
+  // slot layout: FP32 rows of W+4 floats; halves: rows of W+3 pairs (one
+  // dword: two halves); H+4 rows.  Rows padded to 256 bytes.
+  const uint64_t elem  = 4;
   const uint64_t cols  = cfg->precision == LH_PREC_F32 ? cfg->W + 4 : cfg->W + 3;
-  const uint64_t rows  = cfg->precision == LH_PREC_F32 ? cfg->H + 4 : cfg->H + 3;
+  const uint64_t rows  = cfg->H + 4;
   c->rowBytes          = (uint32_t)((cols * elem + 255) / 256 * 256);
   c->slotRecords       = (uint32_t)(c->rowBytes * rows);
   c->slotBytes         = ((uint64_t)c->slotRecords + 4095) / 4096 * 4096;
@@ -1133,7 +1130,7 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
   maxabs_kernel<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, c->stream>>>(src, n,
       c->dMax + slot);
   const int cols = g.precision == LH_PREC_F32 ? (int)g.W + 4 : (int)g.W + 3;
-  const int rows = g.precision == LH_PREC_F32 ? (int)g.H + 4 : (int)g.H + 3;
+  const int rows = (int)g.H + 4;
   dim3 grid((cols + 255) / 256, rows);
   uint8_t *dst = c->dSlots + (uint64_t)slot * c->slotBytes;
   switch (g.precision) {
