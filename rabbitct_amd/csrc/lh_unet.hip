@@ -11,8 +11,9 @@
 // K is packed in 32-wide chunks so that each chunk reads one wave-uniform tap
 // (or one of two taps, chosen by a lane bit):
 //   section 1: for each tap, for each full block of 32 input channels;
-//   section 2 (input channels = 16 mod 32): the last 16 channels of taps
-//   (2p, 2p+1) share one chunk, p = 0..13 (tap 27 is zero).
+//   section 2 (input channels = 16 mod 32): the last 16 channels of two
+//   taps of one z offset, (dz, e) and (dz, e+1) with e = 3 dy + dx even,
+//   share one chunk: 5 chunks per dz (the tenth tap of each is zero).
 // Lane l holds A rows (voxel l%16) and B columns (channel l%16) for the 8
 // K entries 8*(l/16)..+7 of a chunk: 8 consecutive channels of one tap, one
 // 16-byte load each.  The backward-data pass is the same kernel with the taps
@@ -54,13 +55,14 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 // Number of 32-wide K chunks and the chunk/entry of (tap, ci) for C input
 // channels (C a multiple of 16).
-__host__ __device__ __forceinline__ int kchunks(int C) { return 27 * (C / 32) + ((C % 32) ? 14 : 0); }
+__host__ __device__ __forceinline__ int kchunks(int C) { return 27 * (C / 32) + ((C % 32) ? 15 : 0); }
 __host__ __device__ __forceinline__ int kpos(int C, int tap, int ci)
 {
   const int F = C / 32;
   if (ci < 32 * F)
     return (tap * F + ci / 32) * 32 + ci % 32;
-  return (27 * F + tap / 2) * 32 + (tap & 1) * 16 + (ci - 32 * F);
+  const int dz = tap / 9, e = tap % 9;  // e = 3 dy + dx
+  return (27 * F + 5 * dz + e / 2) * 32 + (e & 1) * 16 + (ci - 32 * F);
 }
 
 // ---------------------------------------------------------------------------
@@ -101,34 +103,55 @@ struct ConvLdsArgs {
 constexpr int kHX = 18, kHY = 6;
 
 // Stages the halo of input channels [cbase, cbase + CC) (CC = 32 or 16) as
-// [hz][hy][hx][CC] bfloat16.
+// [hz][hy][hx][CC] bfloat16.  Each thread keeps kStageBatch 16-byte loads in
+// flight before writing them to LDS.
+#ifndef LH_STAGE_BATCH
+#define LH_STAGE_BATCH 8
+#endif
+constexpr int kStageBatch = LH_STAGE_BATCH;
+
 template <int CC, int MT>
 __device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, int cbase, int n,
     int x0, int y0, int z0)
 {
-  constexpr int Q = CC / 8;                // 16-byte units per voxel
-  constexpr int HV = kHX * kHY * (MT + 2); // halo voxels
+  constexpr int Q     = CC / 8;                // 16-byte units per voxel
+  constexpr int HV    = kHX * kHY * (MT + 2);  // halo voxels
+  constexpr int TOTAL = HV * Q;
+  constexpr int ITER  = (TOTAL + 255) / 256;
   const int D = a.D;
-  for (int i = threadIdx.x; i < HV * Q; i += 256) {
-    const int q = i % Q, v = i / Q;
-    const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
-    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-    const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
-                    (unsigned)gz < (unsigned)D;
-    const int c = cbase + 8 * q;
-    const bool first = c < a.c0;
-    const ConvSrc &sr = first ? a.s0 : a.s1;
-    const int cc = first ? c : c - a.c0;
-    const int Ds = sr.half ? (D >> 1) : D;
-    const int sx = sr.half ? (gx >> 1) : gx, sy = sr.half ? (gy >> 1) : gy,
-              sz = sr.half ? (gz >> 1) : gz;
-    const uint32_t off = in ? (uint32_t)(((((uint32_t)n * Ds + sz) * Ds + sy) * Ds + sx) *
-                                         (uint32_t)sr.ld + sr.ofs + cc) * 2u
-                            : kMissOffset;
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
-    const u32x4 val = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-    *reinterpret_cast<u32x4 *>(lds + v * CC + 8 * q) = val;
+#pragma unroll 1
+  for (int b0 = 0; b0 < ITER; b0 += kStageBatch) {
+    u32x4 val[kStageBatch];
+#pragma unroll
+    for (int k = 0; k < kStageBatch; ++k) {
+      const int i = threadIdx.x + (b0 + k) * 256;
+      if (b0 + k < ITER && i < TOTAL) {
+        const int q = i % Q, v = i / Q;
+        const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+                        (unsigned)gz < (unsigned)D;
+        const int c = cbase + 8 * q;
+        const bool first = c < a.c0;
+        const ConvSrc &sr = first ? a.s0 : a.s1;
+        const int cc = first ? c : c - a.c0;
+        const int Ds = sr.half ? (D >> 1) : D;
+        const int sx = sr.half ? (gx >> 1) : gx, sy = sr.half ? (gy >> 1) : gy,
+                  sz = sr.half ? (gz >> 1) : gz;
+        const uint32_t off = in ? (uint32_t)(((((uint32_t)n * Ds + sz) * Ds + sy) * Ds + sx) *
+                                             (uint32_t)sr.ld + sr.ofs + cc) * 2u
+                                : kMissOffset;
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+        val[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kStageBatch; ++k) {
+      const int i = threadIdx.x + (b0 + k) * 256;
+      if (b0 + k < ITER && i < TOTAL)
+        *reinterpret_cast<u32x4 *>(lds + (i / Q) * CC + 8 * (i % Q)) = val[k];
+    }
   }
 }
 
@@ -185,23 +208,59 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       __syncthreads();
     stage_halo<32, MT>(a, lds, 32 * pass, n, x0, y0, z0);
     __syncthreads();
-    bf16x8 Bn[NT];
-    load_b(Bn, wrow, wtile, pass);
-    for (int tap = 0; tap < 27; ++tap) {
-      bf16x8 B[NT];
+    if constexpr (NT <= 2) {
+      // Input planes outer: the A operand of halo plane p at (dy, dx) serves
+      // output rows p - dz for dz = 0, 1, 2 (taps 9 dz + 3 dy + dx), so each
+      // is read from LDS once instead of up to three times.
+      bf16x8 Bn[3][NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        B[t] = Bn[t];
-      if (tap + 1 < 27)
-        load_b(Bn, wrow, wtile, (tap + 1) * F + pass);
-      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-      const uint16_t *base = lds + ((dz * kHY + w + dy) * kHX + m + dx) * 32 + 8 * kg;
+      for (int dz = 0; dz < 3; ++dz)
+        load_b(Bn[dz], wrow, wtile, (9 * dz) * F + pass);
+      for (int dyx = 0; dyx < 9; ++dyx) {
+        bf16x8 B[3][NT];
 #pragma unroll
-      for (int r = 0; r < MT; ++r) {
-        const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + r * RS32));
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            B[dz][t] = Bn[dz][t];
+        if (dyx + 1 < 9)
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz)
+            load_b(Bn[dz], wrow, wtile, (9 * dz + dyx + 1) * F + pass);
+        const int dy = dyx / 3, dx = dyx % 3;
+        const uint16_t *base = lds + ((w + dy) * kHX + m + dx) * 32 + 8 * kg;
+#pragma unroll
+        for (int p = 0; p < MT + 2; ++p) {
+          const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + p * RS32));
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz) {
+            const int r = p - dz;
+            if (r >= 0 && r < MT)
+#pragma unroll
+              for (int t = 0; t < NT; ++t)
+                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[dz][t], acc[r][t], 0, 0, 0);
+          }
+        }
+      }
+    } else {
+      bf16x8 Bn[NT];
+      load_b(Bn, wrow, wtile, pass);
+      for (int tap = 0; tap < 27; ++tap) {
+        bf16x8 B[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
-          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+          B[t] = Bn[t];
+        if (tap + 1 < 27)
+          load_b(Bn, wrow, wtile, (tap + 1) * F + pass);
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        const uint16_t *base = lds + ((dz * kHY + w + dy) * kHX + m + dx) * 32 + 8 * kg;
+#pragma unroll
+        for (int r = 0; r < MT; ++r) {
+          const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + r * RS32));
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+        }
       }
     }
   }
@@ -210,29 +269,72 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       __syncthreads();
     stage_halo<16, MT>(a, lds, 32 * F, n, x0, y0, z0);
     __syncthreads();
-    const int tsel = kg >> 1, cofs = 8 * (kg & 1);
-    bf16x8 Bn[NT];
-    load_b(Bn, wrow, wtile, 27 * F);
-    for (int p = 0; p < 14; ++p) {
-      bf16x8 B[NT];
+    // chunk 27F + 5 dz + j: lanes kg < 2 hold tap (dz, 2j), kg >= 2 tap
+    // (dz, 2j+1) (e = 9 is the zero pad); each lane reads 8 of the 16 channels
+    const int half = kg >> 1, cofs = 8 * (kg & 1);
+    if constexpr (NT <= 2) {
+      // input planes outer, as in the 32-channel passes
+      bf16x8 Bn[3][NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        B[t] = Bn[t];
-      if (p + 1 < 14)
-        load_b(Bn, wrow, wtile, 27 * F + p + 1);
-      const int tap = 2 * p + tsel;
-      const int tp = tap < 27 ? tap : 26;    // tap 27 reads a valid address, zeroed below
-      const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
-      const uint16_t *base = lds + ((dz * kHY + w + dy) * kHX + m + dx) * 16 + cofs;
+      for (int dz = 0; dz < 3; ++dz)
+        load_b(Bn[dz], wrow, wtile, 27 * F + 5 * dz);
+      for (int j = 0; j < 5; ++j) {
+        bf16x8 B[3][NT];
 #pragma unroll
-      for (int r = 0; r < MT; ++r) {
-        uint4 av = *reinterpret_cast<const uint4 *>(base + r * RS16);
-        if (tap >= 27)
-          av = make_uint4(0, 0, 0, 0);
-        const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+        for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+          for (int t = 0; t < NT; ++t)
+            B[dz][t] = Bn[dz][t];
+        if (j + 1 < 5)
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz)
+            load_b(Bn[dz], wrow, wtile, 27 * F + 5 * dz + j + 1);
+        const int e = 2 * j + half;
+        const bool pad = e >= 9;
+        const int ee = pad ? 8 : e;        // the pad reads a valid address, zeroed below
+        const uint16_t *base = lds + ((w + ee / 3) * kHX + m + ee % 3) * 16 + cofs;
+#pragma unroll
+        for (int p = 0; p < MT + 2; ++p) {
+          uint4 av = *reinterpret_cast<const uint4 *>(base + p * RS16);
+          if (pad)
+            av = make_uint4(0, 0, 0, 0);
+          const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz) {
+            const int r = p - dz;
+            if (r >= 0 && r < MT)
+#pragma unroll
+              for (int t = 0; t < NT; ++t)
+                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[dz][t], acc[r][t], 0, 0, 0);
+          }
+        }
+      }
+    } else {
+      bf16x8 Bn[NT];
+      load_b(Bn, wrow, wtile, 27 * F);
+      for (int dz = 0; dz < 3; ++dz) {
+        for (int j = 0; j < 5; ++j) {
+          bf16x8 B[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            B[t] = Bn[t];
+          if (5 * dz + j + 1 < 15)
+            load_b(Bn, wrow, wtile, 27 * F + 5 * dz + j + 1);
+          const int e = 2 * j + half;
+          const bool pad = e >= 9;
+          const int ee = pad ? 8 : e;
+          const uint16_t *base = lds + ((dz * kHY + w + ee / 3) * kHX + m + ee % 3) * 16 + cofs;
+#pragma unroll
+          for (int r = 0; r < MT; ++r) {
+            uint4 av = *reinterpret_cast<const uint4 *>(base + r * RS16);
+            if (pad)
+              av = make_uint4(0, 0, 0, 0);
+            const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+              acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+          }
+        }
       }
     }
   }
