@@ -58,7 +58,7 @@ class Scan:
     phantom: EllipsoidPhantom
     step: int = SPARSE_STEP
     n_held_out: int = 2
-    fp_stride: int = 4
+    fp_stride: int = 8
     ctx: lolahip.Context = field(default=None, repr=False)
     keep_images: bool = False
 
