@@ -98,6 +98,10 @@ struct ConvLdsArgs {
   // sum_c hw[c] y[v][c]; y itself is written only when non-null
   const float *hin, *hw, *hb;
   float *hout;
+  // fused 2x2x2 average pooling of the output (ReLU'd, bfloat16) into a
+  // half-resolution tensor [voxel][ldp] at channel offset pofs; null: none
+  uint16_t *pool;
+  int ldp, pofs;
 };
 
 constexpr int kHX = 18, kHY = 6;
@@ -385,6 +389,38 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       }
     }
   }
+  if (a.pool) {
+    // the tile (16 x 4 x MT, all even) holds whole 2x2x2 cells; wave w's rows
+    // are at y0 + w, so a cell reads the buffers of waves 2Y and 2Y + 1
+    __syncthreads();
+    constexpr int C8 = COUT / 8;
+    constexpr int CELLS = 8 * 2 * (MT / 2);     // x cells, y cells, z cells
+    const int Dh = D >> 1;
+    for (int u = threadIdx.x; u < CELLS * C8; u += 256) {
+      const int c8 = u % C8, cell = u / C8;
+      const int cx = cell % 8, cy = (cell / 8) % 2, cz = cell / 16;
+      float sum[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int zz = 2 * cz + (q >> 2), yy = 2 * cy + ((q >> 1) & 1), xx = 2 * cx + (q & 1);
+        const uint4 v = *reinterpret_cast<const uint4 *>(lds + (size_t)yy * MT * 16 * COUT +
+                                                        (zz * 16 + xx) * COUT + 8 * c8);
+        const uint32_t w4[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          sum[2 * k] += __uint_as_float(w4[k] << 16);
+          sum[2 * k + 1] += __uint_as_float(w4[k] & 0xffff0000u);
+        }
+      }
+      uint32_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        o[k] = (uint32_t)f2bf(0.125f * sum[2 * k]) | ((uint32_t)f2bf(0.125f * sum[2 * k + 1]) << 16);
+      const size_t vo = ((((size_t)n * Dh + (z0 >> 1) + cz) * Dh + (y0 >> 1) + cy) * Dh) +
+                        (x0 >> 1) + cx;
+      *reinterpret_cast<uint4 *>(a.pool + vo * a.ldp + a.pofs + 8 * c8) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
   if constexpr (COUT == 16) {
     if (a.hout) {
       // one voxel per lane: its 16 channels (bfloat16) dotted with the head
@@ -611,37 +647,6 @@ __global__ void relu_bwd_kernel(uint16_t *g, int ldg, int gofs, const uint16_t *
     const size_t v = i / C, c = i % C;
     if (!(bf2f(y[v * ldy + yofs + c]) > 0.0f))
       g[v * ldg + gofs + c] = 0;
-  }
-}
-
-// 2x2x2 average pooling, channel slices in and out (C, strides and offsets
-// multiples of 8: one thread per 8 channels of an output voxel)
-__global__ void pool_kernel(const uint16_t *x, int ldx, int xofs, uint16_t *y, int ldy, int yofs,
-    int C, int Do, int n)
-{
-  const int C8 = C / 8;
-  const size_t total = (size_t)n * Do * Do * Do * C8;
-  const int Di = 2 * Do;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    const int c = 8 * (int)(i % C8);
-    const size_t v = i / C8;
-    const int xo = v % Do, yo = (v / Do) % Do, zo = (v / (Do * Do)) % Do, b = v / ((size_t)Do * Do * Do);
-    float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
-    for (int q = 0; q < 8; ++q) {
-      const size_t vi = (((size_t)b * Di + 2 * zo + (q >> 2)) * Di + 2 * yo + ((q >> 1) & 1)) * Di + 2 * xo + (q & 1);
-      const uint4 u = *reinterpret_cast<const uint4 *>(x + vi * ldx + xofs + c);
-      const uint32_t w4[4] = { u.x, u.y, u.z, u.w };
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        s[2 * k] += __uint_as_float(w4[k] << 16);
-        s[2 * k + 1] += __uint_as_float(w4[k] & 0xffff0000u);
-      }
-    }
-    uint32_t o[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      o[k] = (uint32_t)f2bf(0.125f * s[2 * k]) | ((uint32_t)f2bf(0.125f * s[2 * k + 1]) << 16);
-    *reinterpret_cast<uint4 *>(y + v * ldy + yofs + c) = make_uint4(o[0], o[1], o[2], o[3]);
   }
 }
 
@@ -951,10 +956,21 @@ struct HeadFuse {
   float *out = nullptr;
 };
 
+struct PoolFuse {
+  uint16_t *p = nullptr;
+  int ld = 0, ofs = 0;
+};
+
 int conv_fwd(hipStream_t s, const Conv &L, ConvIn in, Act out, int D, int n, bool relu,
-    bool bias, bool accumulate, bool backward, HeadFuse head = HeadFuse())
+    bool bias, bool accumulate, bool backward, HeadFuse head = HeadFuse(),
+    PoolFuse pool = PoolFuse())
 {
   ConvLdsArgs a;
+  a.pool = pool.p;
+  a.ldp = pool.ld;
+  a.pofs = pool.ofs;
+  if (pool.p && (pool.ld % 8 || pool.ofs % 8))
+    return lh_fail(LH_E_ARG, "conv: pooled output stride/offset must be multiples of 8");
   a.hin = head.in;
   a.hw = head.w;
   a.hb = head.b;
@@ -1119,7 +1135,7 @@ int forward(LhUnet *u, const float *in, int n, int D, bool training)
     return rc;
   hipStream_t s = u->stream;
   const int D1 = D, D2 = D / 2, D3 = D / 4;
-  const int n1 = n * D1 * D1 * D1, n2 = n * D2 * D2 * D2, n3 = n * D3 * D3 * D3;
+  const int n1 = n * D1 * D1 * D1;
   auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
   if (training)
     pack_input_kernel<<<grid_for((size_t)n1 * 16), 256, 0, s>>>(in, u->x16, (size_t)n1);
@@ -1129,11 +1145,19 @@ int forward(LhUnet *u, const float *in, int n, int D, bool training)
         tiles % 8 == 0 ? (int)(tiles / 8) : 0);
   }
 #define FW(L, i, o, Dd) if ((rc = conv_fwd(s, u->c[L], i, o, Dd, n, true, true, false, false)) != LH_OK) return rc
-  FW(1, A(u->a1, 16), A(u->e1, 16), D1);
-  pool_kernel<<<grid_for((size_t)n2 * 2), 256, 0, s>>>(u->e1, 16, 0, u->p1, 16, 0, 16, D2, n);
+  // the encoders' convolutions also write the pooled tensors
+  PoolFuse pf1, pf2;
+  pf1.p = u->p1;
+  pf1.ld = 16;
+  pf2.p = u->p2;
+  pf2.ld = 32;
+  if ((rc = conv_fwd(s, u->c[1], plain_in(A(u->a1, 16)), A(u->e1, 16), D1, n, true, true, false,
+           false, HeadFuse(), pf1)) != LH_OK)
+    return rc;
   FW(2, A(u->p1, 16), A(u->a2, 32), D2);
-  FW(3, A(u->a2, 32), A(u->e2, 32), D2);
-  pool_kernel<<<grid_for((size_t)n3 * 4), 256, 0, s>>>(u->e2, 32, 0, u->p2, 32, 0, 32, D3, n);
+  if ((rc = conv_fwd(s, u->c[3], plain_in(A(u->a2, 32)), A(u->e2, 32), D2, n, true, true, false,
+           false, HeadFuse(), pf2)) != LH_OK)
+    return rc;
   FW(4, A(u->p2, 32), A(u->a3, 64), D3);
   FW(5, A(u->a3, 64), A(u->b3, 64), D3);
   FW(6, dec_in(u, 2), A(u->d2a, 32), D2);
