@@ -46,6 +46,7 @@ from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry, voxel_size  # n
 
 METRIC = "GUPS at L=512 plain vs fused/bf16; volumes/s sparse-view+CNN at fixed RMSE"
 HBM_PEAK = 8.0e12
+MFMA_BF16_PEAK = 2500.0   # TFLOP/s, dense
 SEED = 512
 EVAL_SCAN = 200000   # sparse-view scan index (training uses 0..249, validation 100000..)
 
@@ -169,6 +170,14 @@ def sparse_cnn(args):
         "rmse_sparse_only_vs_full": learned.rmse_of_range(sparse, full),
         "fallback": res["fallback"], "residual": res["residual"], "threshold": rec.threshold,
     }
+    # the U-Net stage against the dense BF16 MFMA peak: executed FLOPs (K
+    # padding included) over the stage's wall time
+    from rabbitct_amd import unet
+    flops = 2.0 * unet.executed_macs_per_voxel() * L ** 3
+    tf = flops / (res["ms"]["cnn"] * 1e-3) / 1e12
+    out["cnn_roofline"] = {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK,
+                           "unit": "TFLOP/s", "frac": round(tf / MFMA_BF16_PEAK, 4),
+                           "flops_per_volume": flops}
     log(f"sparse_cnn: full plain {ms_full:.2f} ms, sparse+CNN {ms_pipe:.2f} ms "
         f"({out['speedup']}x), rmse {out['rmse_sparse_cnn_vs_full']:.4g} "
         f"(sparse only {out['rmse_sparse_only_vs_full']:.4g}), fallback {res['fallback']}")

@@ -527,12 +527,16 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         };
         // groups of four updates are fenced so the scheduler keeps at most four
         // updates' taps in flight per lane
-        if (__all(rlo == 0 && rhi == R)) {
+        // with VECTORSIZE-aligned ranges a run is all in or all out; lanes
+        // whose run is out are masked off the unrolled loop
+        if (__all(!active || (rlo == 0 && rhi == R))) {
+          if (active) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            acc[r] += update(r);
-            if ((r % LH_FENCE) == LH_FENCE - 1)
-              __builtin_amdgcn_sched_barrier(0);
+            for (int r = 0; r < R; ++r) {
+              acc[r] += update(r);
+              if ((r % LH_FENCE) == LH_FENCE - 1)
+                __builtin_amdgcn_sched_barrier(0);
+            }
           }
         } else {
 #pragma unroll

@@ -44,8 +44,9 @@ def slot_order(n_views: int, step: int = SPARSE_STEP) -> np.ndarray:
 
 def held_out_views(n_views: int, step: int = SPARSE_STEP, count: int = 2) -> np.ndarray:
     """`count` views spread over the arc, none in the sparse subset."""
-    base = np.linspace(0, n_views, count, endpoint=False).astype(int) + step // 2
-    return np.minimum(base, n_views - 1)
+    base = np.linspace(0, n_views, count, endpoint=False).astype(int)
+    v = (base // step) * step + max(1, step // 2)     # the middle of a step group
+    return np.where(v < n_views, v, v - step + 1)
 
 
 @dataclass

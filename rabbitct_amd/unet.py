@@ -26,6 +26,26 @@ def param_count() -> int:
     return sum(co * 27 * ci + co for ci, co in LAYERS) + 17
 
 
+# resolution level (0 full, 1 half, 2 quarter) of each layer
+LEVELS = (0, 0, 1, 1, 2, 2, 1, 1, 0, 0)
+
+
+def _kchunks(c: int) -> int:
+    """32-wide K chunks of one 3x3x3 convolution with c input channels (the
+    packing of lh_unet.hip): 27 per 32-channel block, 15 for a 16-channel
+    tail."""
+    return 27 * (c // 32) + (15 if c % 32 else 0)
+
+
+def executed_macs_per_voxel() -> float:
+    """MACs the forward pass executes per full-resolution voxel, K padding
+    included (the first layer's 27 taps of one channel run as one chunk)."""
+    m = 32.0 * 16
+    for (ci, co), lvl in zip(LAYERS[1:], LEVELS[1:]):
+        m += 32 * _kchunks(ci) * co / 8 ** lvl
+    return m
+
+
 class DeviceBuffer:
     """A device allocation owned by this object (lhDeviceAlloc/lhDeviceFree)."""
 
