@@ -303,3 +303,36 @@ def test_rct_file_ingest_equals_host_upload(tmp_path, golden_border):
             with pytest.raises(lolahip.LolaHipError):
                 ctx.load_rct(p, 5, N, first_slot=0)          # past the end of the file
     assert np.array_equal(vols[0], vols[1])
+
+
+def test_reference_runner_verifies_lolahip(tmp_path, golden_border):
+    """The runner's own -c verification on LolaHIP's volume: against the HU
+    volume of the FP64 oracle it prints the numbers the oracle's metric gives
+    for LolaHIP's -o output, and HU rounding flips are the only errors."""
+    import re
+    runner = orc.ref_runner(orc.host_isa(), lolahip=True) or orc.ref_runner("avx2", lolahip=True)
+    if runner is None:
+        pytest.skip("reference runner with LolaHIP not built")
+    g = golden_border
+    N, H, W = g["images"].shape
+    h0, h1 = (float(x) for x in g["hu"])
+    hdr = rct.RctHeader(W, H, N, (h0, h1))
+    rct.write_rct(tmp_path / "in.rct", hdr, g["matrices"], g["images"])
+    rct.write_geometry(tmp_path / "geo.rct", hdr, g["matrices"])
+    rct.write_clip_file(tmp_path / "clip16.rct", g["clip16"])
+    ref = orc.backproject(L, voxel_size(L), volume_origin(L), g["matrices"], g["images"],
+                          g["clip16"]).astype(np.float32)
+    hu_ref = orc.hu_convert(ref, h0, h1).reshape(-1)
+    hu_ref.tofile(tmp_path / "Reference128.vol")
+    r = subprocess.run([runner, "-i", "in.rct", "-a", "geo.rct", "-m", "LolaHIP", "-s", "128",
+                        "-b", "4", "-C", "clip16.rct", "-o", "hip.vol", "-c", "Reference128.vol"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = lambda key: float(re.search(r"(?m)^" + key + r": ([0-9.eE+-]+|INF)", r.stdout).group(1)
+                          .replace("INF", "inf"))
+    vol = np.fromfile(tmp_path / "hip.vol", dtype=np.float32)
+    m = orc.hu_metric(vol, hu_ref, h0, h1)
+    print(f"\n-c on LolaHIP vs FP64 HU: mse {f('Mean Squared Error')} max "
+          f"{f('Max. Absolute Error')}")
+    assert f("Mean Squared Error") == pytest.approx(m["mse"], rel=1e-5, abs=1e-12)
+    assert f("Max. Absolute Error") == m["max"] <= 1.0
