@@ -291,12 +291,14 @@ constexpr int kTileZ = 8;   // z extent of the staged tile
 struct StageBox {
   int iu0, iv0;   // footprint origin (pixels)
   int fw, fh;     // footprint size
-  float lou, hiu, lov, hiv; // clamp bounds of the voxel coordinates
-  bool fits;
+  bool fits;      // staged in LDS (else L2 gathers)
+  bool interior;  // the detector edge does not clip the footprint and every
+                  // coordinate of the tile is >= 1 pixel: no clamp is needed
 };
 
 // Footprint boxes are computed once per (slot, tile) by box_kernel and packed
-// as int16 {iu0, iv0, fw, fh}; fw = 0 marks "no box" (the L2 fallback).
+// as int16 {iu0, iv0, fw | interior << 15, fh}; fw = 0 marks "no box" (the
+// L2 fallback).
 __device__ __forceinline__ StageBox load_box(const BpArgs &A, int slot, int tile)
 {
   const uint64_t i = (uint64_t)slot * (uint64_t)(A.tilesX * A.tilesY * A.tilesZ) + tile;
@@ -304,22 +306,12 @@ __device__ __forceinline__ StageBox load_box(const BpArgs &A, int slot, int tile
       (const __attribute__((address_space(4))) int *)(uintptr_t)A.boxes + 2 * i;
   const int qx = q[0], qy = q[1];
   StageBox b;
-  b.iu0  = (int)(short)(qx & 0xffff);
-  b.iv0  = qx >> 16;
-  b.fw   = qy & 0xffff;
-  b.fh   = qy >> 16;
-  b.fits = b.fw >= 2 && b.fh >= 2 && b.fw <= kFW && b.fh <= kFH;
-  if (b.fits) {
-    b.lou = (float)b.iu0;
-    b.hiu = (float)(b.iu0 + b.fw - 2);
-    b.lov = (float)b.iv0;
-    b.hiv = (float)(b.iv0 + b.fh - 2);
-  } else {
-    b.lou = -2.0f;
-    b.hiu = A.Wf;
-    b.lov = -2.0f;
-    b.hiv = A.Hf;
-  }
+  b.iu0      = (int)(short)(qx & 0xffff);
+  b.iv0      = qx >> 16;
+  b.fw       = qy & 0x7fff;
+  b.interior = (qy & 0x8000) != 0;
+  b.fh       = qy >> 16;
+  b.fits     = b.fw >= 2 && b.fh >= 2 && b.fw <= kFW && b.fh <= kFH;
   return b;
 }
 
@@ -328,6 +320,11 @@ __device__ __forceinline__ StageBox load_box(const BpArgs &A, int slot, int tile
 // keep [floor(min) - 1, floor(max) + 3): the taps of a coordinate t lie in
 // [trunc(t), trunc(t) + 1] within [floor(t), floor(t) + 2], plus one pixel of
 // margin on each side for the FP32 rounding of the per-voxel coordinates.
+// The projection of the tile (a box in front of the source) is the convex hull
+// of its corners' projections, so every voxel's coordinate lies in the box.
+// "interior": no corner was clamped, all lie at >= 1 pixel and below W (H),
+// so the clamp is the identity for the whole tile and the coordinates are
+// positive (truncation is the floor).
 __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, int nTiles,
     int tilesX, int tilesY, int R, float O_L, float R_L, int W, int H, int2 *boxes)
 {
@@ -339,6 +336,7 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
   const int tx = t % tilesX, ty = (t / tilesX) % tilesY, tz = t / (tilesX * tilesY);
   const double O = O_L, RL = R_L;
   double umin = 1e30, umax = -1e30, vmin = 1e30, vmax = -1e30;
+  bool interior = true;
   for (int c = 0; c < 8; ++c) {
     const double x = O + (double)(tx * R + ((c & 1) ? R - 1 : 0)) * RL;
     const double y = O + (double)(ty * kLYs + ((c & 2) ? kLYs - 1 : 0)) * RL;
@@ -346,6 +344,7 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
     const double hw = fma(a[2], x, fma(a[5], y, fma(a[8], z, a[11])));
     const double u  = fma(a[0], x, fma(a[3], y, fma(a[6], z, a[9]))) / hw;
     const double v  = fma(a[1], x, fma(a[4], y, fma(a[7], z, a[10]))) / hw;
+    interior = interior && hw > 0.0 && u >= 1.0 && v >= 1.0 && u < (double)W && v < (double)H;
     const double uc = fmin(fmax(u, -2.0), (double)W), vc = fmin(fmax(v, -2.0), (double)H);
     umin = fmin(umin, uc);
     umax = fmax(umax, uc);
@@ -358,7 +357,8 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
   const int iv1 = min((int)floor(vmax) + 3, H + 2);
   int2 q;
   q.x = (int)(uint16_t)(int16_t)iu0 | (int)((uint32_t)(uint16_t)(int16_t)iv0 << 16);
-  q.y = (int)(uint16_t)(iu1 - iu0) | (int)((uint32_t)(uint16_t)(iv1 - iv0) << 16);
+  q.y = (int)((uint32_t)(iu1 - iu0) & 0x7fffu) | (interior ? 0x8000 : 0) |
+        (int)((uint32_t)(uint16_t)(iv1 - iv0) << 16);
   boxes[(uint64_t)slot * (uint64_t)nTiles + t] = q;
 }
 
@@ -366,31 +366,30 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
 // FP32: each buffer_load_dwordx4 ... lds moves two rows of 128 pixels (lane l
 // lands 16 bytes at l*16, so lanes 32-63 fill the next LDS row); wave w copies
 // row pairs 2w, 2w+8, ... (a half image holds one dword per element too: the
-// horizontal pair p(i,j), p(i+1,j)).  Other layouts: one dword per lane,
+// horizontal pair p(i,j), p(i+1,j)).  laneOff is the lane's byte offset within
+// a row pair, fixed per tile; the loop control is wave-uniform and only the
+// lower half-wave copies a last odd row.  Other layouts: one dword per lane,
 // pixels [iu0 + c0, iu0 + c0 + 64) of a row per instruction, wave w copying
 // rows w, w+4, ....  Columns at or beyond fw are copied too and never
 // read; past the end of the slot the buffer bounds return zero.  Completion is
 // awaited by the caller (s_waitcnt vmcnt(0) and a barrier).
 template <int ELEM>
 __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StageBox &b,
-    int pitch, float *buf)
+    int pitch, float *buf, int wv, uint32_t laneOff, bool loHalf)
 {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv   = threadIdx.x / kWave;
   const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * ELEM;
   if constexpr (ELEM == 4 && kPitch == 128) {
-    // 16 bytes (4 pixels) per lane: lanes 0-31 stage row r, lanes 32-63 row
-    // r+1, which lands 512 bytes (one LDS row) further; rows past fh are
-    // masked off.  Columns past fw (up to 128) are staged and never read.
-    const int half = lane >> 5, col = (lane & 31) * 4;
-    for (int r0 = 2 * wv; r0 < b.fh; r0 += 2 * kWaves) {
-      const int r = r0 + half;
-      if (r < b.fh)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-            (__attribute__((address_space(3))) void *)(buf + r0 * kPitch), 16,
-            (int)(base + (uint32_t)(r * pitch + col) * 4u), 0, 0, 0);
-    }
+    uint32_t off        = base + laneOff + (uint32_t)(2 * wv * pitch) * 4u;
+    const uint32_t step = (uint32_t)(2 * kWaves * pitch) * 4u;
+    int r0              = 2 * wv;
+    for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += step)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * kPitch), 16, (int)off, 0, 0, 0);
+    if (r0 < b.fh && loHalf)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * kPitch), 16, (int)off, 0, 0, 0);
   } else {
+    const int lane = threadIdx.x & (kWave - 1);
     for (int r = wv; r < b.fh; r += kWaves) {
       const uint32_t row = base + (uint32_t)(r * pitch) * ELEM;
       for (int c0 = 0; c0 < b.fw; c0 += kWave) {
@@ -402,14 +401,68 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
   }
 }
 
+// One voxel-view update from the LDS footprint (LolaOPT.c:145-170): sBase[v *
+// kPitch + u] is footprint pixel (u, v).  CLAMP: coordinates clamped to the
+// box [lou, hiu] x [lov, hiv], truncation toward zero.  Without CLAMP (interior
+// boxes) the coordinates are positive and inside the box, so trunc is the
+// floor, x - trunc(x) is fract(x) exactly and the conversion truncates.
+template <int PREC, bool CLAMP>
+__device__ __forceinline__ float lds_update(const float *sBase, float u, float v, float w,
+    float lou, float hiu, float lov, float hiv)
+{
+  const float rw = __builtin_amdgcn_rcpf(w);
+  float ix = u * rw, iy = v * rw;
+  float ax, ay;
+  const float *q;
+  if constexpr (CLAMP) {
+    ix = __builtin_amdgcn_fmed3f(ix, lou, hiu);
+    iy = __builtin_amdgcn_fmed3f(iy, lov, hiv);
+    const float fx = __builtin_truncf(ix);
+    const float fy = __builtin_truncf(iy);
+    // fy*kPitch + fx is an exact integer in FP32 (|.| < 2^24): one conversion
+    q  = sBase + (int)fmaf(fy, (float)kPitch, fx);
+    ax = ix - fx;
+    ay = iy - fy;
+  } else {
+    ax = __builtin_amdgcn_fractf(ix);
+    ay = __builtin_amdgcn_fractf(iy);
+    // byte offset row * pitch + 4 * column: one 24-bit multiply-add after the shift
+    q = reinterpret_cast<const float *>(reinterpret_cast<const char *>(sBase) +
+        __mul24((int)iy, kPitch * 4) + ((int)ix << 2));
+  }
+  float b, t;
+  if constexpr (PREC == LH_PREC_F32) {
+    const float p00 = q[0], p10 = q[1];
+    const float p01 = q[kPitch], p11 = q[kPitch + 1];
+    b = fmaf(ax, p10 - p00, p00);
+    t = fmaf(ax, p11 - p01, p01);
+  } else {
+    // one dword per pixel: the horizontal pair (p(i,j), p(i+1,j))
+    const uint32_t *qq = reinterpret_cast<const uint32_t *>(q);
+    const uint32_t q0 = qq[0], q1 = qq[kPitch];
+    if constexpr (PREC == LH_PREC_F16) {
+      // (1-ax) p0 + ax p1 with the FP16 operands widened inside the FMA
+      b = fmaf(ax, f16_hi(q0), fmaf(-ax, f16_lo(q0), f16_lo(q0)));
+      t = fmaf(ax, f16_hi(q1), fmaf(-ax, f16_lo(q1), f16_lo(q1)));
+    } else {
+      const float p00 = bf16_lo(q0), p10 = bf16_hi(q0);
+      const float p01 = bf16_lo(q1), p11 = bf16_hi(q1);
+      b = fmaf(ax, p10 - p00, p00);
+      t = fmaf(ax, p11 - p01, p01);
+    }
+  }
+  return fmaf(ay, t - b, b) * (rw * rw);
+}
+
 template <int R, int PREC>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
 {
-  constexpr int ELEM = 4;  // FP32 pixel or half pair: one dword per element
+  constexpr int ELEM    = 4;         // FP32 pixel or half pair: one dword per element
+  constexpr float kPark = 0x1p100f;  // w of a parked lane: (1/w)^2 underflows to +0
   const int tile = tx + A.tilesX * (ty + A.tilesY * tz);
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv   = threadIdx.x / kWave;
+  const int wv   = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int x0   = tx * R;
   const int y0   = ty * kLYs;
   const int z0   = tz * kTileZ;
@@ -425,6 +478,9 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   const uint64_t LL   = (uint64_t)L * (uint64_t)L;
   const int pitch     = (int)(A.rowBytes / ELEM);
   const int n         = A.nViews;
+  // the lane's byte offset within a staged row pair (16 bytes per lane)
+  const uint32_t laneOff = (uint32_t)(((lane >> 5) * pitch + (lane & 31) * 4) * ELEM);
+  const bool loHalf      = lane < 32;
 
   auto range_of = [&](int v) -> int {
     return A.clip ? A.ranges[(uint64_t)(view_params(A, A.firstSlot + v).id - 1) * LL + line]
@@ -449,10 +505,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   if (lane == 0)
     sAct[wv] = act0 ? 1 : 0;
   __syncthreads();
-  bool blockAct = (sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
+  bool blockAct = __builtin_amdgcn_readfirstlane(sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
   StageBox box  = load_box(A, A.firstSlot, tile);
   if (blockAct && box.fits)
-    stage_issue<ELEM>(slot_rsrc(A, A.firstSlot), box, pitch, sImg);
+    stage_issue<ELEM>(slot_rsrc(A, A.firstSlot), box, pitch, sImg, wv, laneOff, loHalf);
 
   for (int v = 0; v < n; ++v) {
     const int slot      = A.firstSlot + v;
@@ -465,7 +521,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       sAct[((v + 1) & 1) * kWaves + wv] = actNext ? 1 : 0;
     __syncthreads();
     const int *an = sAct + ((v + 1) & 1) * kWaves;
-    const bool blockActNext = (an[0] | an[1] | an[2] | an[3]) != 0;
+    const bool blockActNext = __builtin_amdgcn_readfirstlane(an[0] | an[1] | an[2] | an[3]) != 0;
     const int rr   = rr_cur;
     rr_cur         = rr_nxt;
     if (v + 2 < n)
@@ -474,7 +530,8 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     if (hasNext) {
       boxNext = load_box(A, slot + 1, tile);
       if (blockActNext && boxNext.fits)
-        stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch, sImg + ((v + 1) & 1) * kPitch * kFH);
+        stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch,
+            sImg + ((v + 1) & 1) * kPitch * kFH, wv, laneOff, loHalf);
     }
 
     const int start   = (int)(short)(rr & 0xffff);
@@ -484,65 +541,57 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     const bool active = rlo < rhi;
     if (blockAct && __any(active)) {
       touched |= active;
-      const float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
-      const float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
-      const float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
+      float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
+      float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
+      float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
       const float du = P.du, dv = P.dv, dw = P.dw;
+      // Voxels outside their line's range are "parked": (u, v, w) = (pu, pv, 1)
+      // * 2^100 puts the coordinate at (pu, pv), inside the footprint half a
+      // pixel from its corner, and the weight (1/w)^2 underflows to +0, so the
+      // update adds exactly zero.  The accumulators then take one straight
+      // path through every branch below.
+      const float pu = box.fits ? (float)box.iu0 + 0.5f : 0.0f;
+      const float pv = box.fits ? (float)box.iv0 + 0.5f : 0.0f;
+      const bool full = __all(!active || (rlo == 0 && rhi == R));
+      if (full && !active) {
+        u0 = pu * kPark;
+        v0 = pv * kPark;
+        w0 = kPark;
+      }
       if (box.fits) {
         // sBase[v*kPitch + u] is footprint pixel (u, v): the window origin folded in
         const float *sBase = sImg + (v & 1) * kPitch * kFH - (box.iv0 * kPitch + box.iu0);
-        auto update = [&](int r) -> float {
-          const float fr = (float)r;
-          const float rw = __builtin_amdgcn_rcpf(fmaf(fr, dw, w0));
-          const float ix = __builtin_amdgcn_fmed3f(fmaf(fr, du, u0) * rw, box.lou, box.hiu);
-          const float iy = __builtin_amdgcn_fmed3f(fmaf(fr, dv, v0) * rw, box.lov, box.hiv);
-          const float fx = __builtin_truncf(ix);
-          const float fy = __builtin_truncf(iy);
-          // fy*kPitch + fx is an exact integer in FP32 (|.| < 2^24): one conversion
-          const int a    = (int)fmaf(fy, (float)kPitch, fx);
-          const float ax = ix - fx, ay = iy - fy;
-          float b, t;
-          if constexpr (PREC == LH_PREC_F32) {
-            const float *q  = sBase + a;
-            const float p00 = q[0], p10 = q[1];
-            const float p01 = q[kPitch], p11 = q[kPitch + 1];
-            b = fmaf(ax, p10 - p00, p00);
-            t = fmaf(ax, p11 - p01, p01);
-          } else {
-            // one dword per pixel: the horizontal pair (p(i,j), p(i+1,j))
-            const uint32_t *q = reinterpret_cast<const uint32_t *>(sBase) + a;
-            const uint32_t q0 = q[0], q1 = q[kPitch];
-            if constexpr (PREC == LH_PREC_F16) {
-              // (1-ax) p0 + ax p1 with the FP16 operands widened inside the FMA
-              b = fmaf(ax, f16_hi(q0), fmaf(-ax, f16_lo(q0), f16_lo(q0)));
-              t = fmaf(ax, f16_hi(q1), fmaf(-ax, f16_lo(q1), f16_lo(q1)));
-            } else {
-              const float p00 = bf16_lo(q0), p10 = bf16_hi(q0);
-              const float p01 = bf16_lo(q1), p11 = bf16_hi(q1);
-              b = fmaf(ax, p10 - p00, p00);
-              t = fmaf(ax, p11 - p01, p01);
-            }
-          }
-          return fmaf(ay, t - b, b) * (rw * rw);
-        };
-        // groups of four updates are fenced so the scheduler keeps at most four
-        // updates' taps in flight per lane
-        // with VECTORSIZE-aligned ranges a run is all in or all out; lanes
-        // whose run is out are masked off the unrolled loop
-        if (__all(!active || (rlo == 0 && rhi == R))) {
-          if (active) {
+        const float lou = (float)box.iu0, hiu = (float)(box.iu0 + box.fw - 2);
+        const float lov = (float)box.iv0, hiv = (float)(box.iv0 + box.fh - 2);
+        // groups of LH_FENCE updates are fenced so the scheduler keeps at most
+        // that many updates' taps in flight per lane
+        if (full && box.interior) {
+          // with VECTORSIZE-aligned ranges a run is all in or all out
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              acc[r] += update(r);
-              if ((r % LH_FENCE) == LH_FENCE - 1)
-                __builtin_amdgcn_sched_barrier(0);
-            }
+          for (int r = 0; r < R; ++r) {
+            const float fr = (float)r;
+            acc[r] += lds_update<PREC, false>(sBase, fmaf(fr, du, u0), fmaf(fr, dv, v0),
+                fmaf(fr, dw, w0), lou, hiu, lov, hiv);
+            if ((r % LH_FENCE) == LH_FENCE - 1)
+              __builtin_amdgcn_sched_barrier(0);
+          }
+        } else if (full) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const float fr = (float)r;
+            acc[r] += lds_update<PREC, true>(sBase, fmaf(fr, du, u0), fmaf(fr, dv, v0),
+                fmaf(fr, dw, w0), lou, hiu, lov, hiv);
+            if ((r % LH_FENCE) == LH_FENCE - 1)
+              __builtin_amdgcn_sched_barrier(0);
           }
         } else {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            if (r >= rlo && r < rhi)
-              acc[r] += update(r);
+            const float fr = (float)r;
+            const bool on  = r >= rlo && r < rhi;
+            acc[r] += lds_update<PREC, true>(sBase, on ? fmaf(fr, du, u0) : pu * kPark,
+                on ? fmaf(fr, dv, v0) : pv * kPark, on ? fmaf(fr, dw, w0) : kPark, lou, hiu,
+                lov, hiv);
             if ((r % LH_FENCE) == LH_FENCE - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
@@ -552,11 +601,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         const __amdgpu_buffer_rsrc_t rs = slot_rsrc(A, slot);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          if (r >= rlo && r < rhi) {
-            const float fr = (float)r;
-            acc[r] += voxel_update<PREC>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0),
-                fmaf(fr, dw, w0));
-          }
+          const float fr = (float)r;
+          const bool on  = r >= rlo && r < rhi;
+          acc[r] += voxel_update<PREC>(rs, A, on ? fmaf(fr, du, u0) : 0.0f,
+              on ? fmaf(fr, dv, v0) : 0.0f, on ? fmaf(fr, dw, w0) : kPark);
           __builtin_amdgcn_sched_barrier(0);
         }
       }
