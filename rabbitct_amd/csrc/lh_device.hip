@@ -102,7 +102,9 @@ struct BpArgs {
   int clip;
   int tilesX, tilesY, tilesZ; // tile grid (x-runs, 64-line groups, 4-slice groups)
   int persistent;
-  const int2 *boxes;       // staged kernel: footprint box per (slot, tile)
+  const int2 *boxes;       // staged kernel: footprint box per (tile, slot), [tile][boxSlots]
+  int boxSlots;            // slots per tile row of boxes
+  uint32_t boxBytes;       // bytes of the boxes array (< 4 GiB)
 };
 
 // Per-view parameters and footprint boxes are read through the constant
@@ -298,13 +300,28 @@ struct StageBox {
 
 // Footprint boxes are computed once per (slot, tile) by box_kernel and packed
 // as int16 {iu0, iv0, fw | interior << 15, fh}; fw = 0 marks "no box" (the
-// L2 fallback).
-__device__ __forceinline__ StageBox load_box(const BpArgs &A, int slot, int tile)
+// L2 fallback).  They are stored [tile][slot], so a tile's boxes for
+// consecutive views share cache lines, and are read with vector loads issued
+// two views ahead (they land under the same vmcnt wait as the footprints):
+// a scalar load would be waited for by every LDS access (lgkmcnt).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const BpArgs &A)
 {
-  const uint64_t i = (uint64_t)slot * (uint64_t)(A.tilesX * A.tilesY * A.tilesZ) + tile;
-  const __attribute__((address_space(4))) int *q =
-      (const __attribute__((address_space(4))) int *)(uintptr_t)A.boxes + 2 * i;
-  const int qx = q[0], qy = q[1];
+  return __builtin_amdgcn_make_buffer_rsrc((void *)A.boxes, (short)0, (int)A.boxBytes,
+      (int)kRsrcWord3);
+}
+
+__device__ __forceinline__ uint2 fetch_box(__amdgpu_buffer_rsrc_t rs, const BpArgs &A, int slot,
+    int tile)
+{
+  const uint32_t off = ((uint32_t)tile * (uint32_t)A.boxSlots + (uint32_t)slot) * 8u;
+  const auto q       = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+  return make_uint2(q[0], q[1]);
+}
+
+__device__ __forceinline__ StageBox decode_box(uint2 raw)
+{
+  const int qx = __builtin_amdgcn_readfirstlane((int)raw.x);
+  const int qy = __builtin_amdgcn_readfirstlane((int)raw.y);
   StageBox b;
   b.iu0      = (int)(short)(qx & 0xffff);
   b.iv0      = qx >> 16;
@@ -326,7 +343,7 @@ __device__ __forceinline__ StageBox load_box(const BpArgs &A, int slot, int tile
 // so the clamp is the identity for the whole tile and the coordinates are
 // positive (truncation is the floor).
 __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, int nTiles,
-    int tilesX, int tilesY, int R, float O_L, float R_L, int W, int H, int2 *boxes)
+    int tilesX, int tilesY, int R, float O_L, float R_L, int W, int H, int nSlots, int2 *boxes)
 {
   const int t    = blockIdx.x * blockDim.x + threadIdx.x;
   const int slot = firstSlot + blockIdx.y;
@@ -359,7 +376,7 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
   q.x = (int)(uint16_t)(int16_t)iu0 | (int)((uint32_t)(uint16_t)(int16_t)iv0 << 16);
   q.y = (int)((uint32_t)(iu1 - iu0) & 0x7fffu) | (interior ? 0x8000 : 0) |
         (int)((uint32_t)(uint16_t)(iv1 - iv0) << 16);
-  boxes[(uint64_t)slot * (uint64_t)nTiles + t] = q;
+  boxes[(uint64_t)t * (uint64_t)nSlots + slot] = q;
 }
 
 // Issues the direct-to-LDS loads of footprint `b` of the slot image into buf.
@@ -506,7 +523,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     sAct[wv] = act0 ? 1 : 0;
   __syncthreads();
   bool blockAct = __builtin_amdgcn_readfirstlane(sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
-  StageBox box  = load_box(A, A.firstSlot, tile);
+  const __amdgpu_buffer_rsrc_t boxRs = box_rsrc(A);
+  StageBox box = decode_box(fetch_box(boxRs, A, A.firstSlot, tile));
+  // box of view v+1, in flight during view v-1
+  uint2 boxRaw = n > 1 ? fetch_box(boxRs, A, A.firstSlot + 1, tile) : make_uint2(0u, 0u);
   if (blockAct && box.fits)
     stage_issue<ELEM>(slot_rsrc(A, A.firstSlot), box, pitch, sImg, wv, laneOff, loHalf);
 
@@ -528,7 +548,9 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       rr_nxt = range_of(v + 2);
     StageBox boxNext = box;
     if (hasNext) {
-      boxNext = load_box(A, slot + 1, tile);
+      boxNext = decode_box(boxRaw);
+      if (v + 2 < n)
+        boxRaw = fetch_box(boxRs, A, slot + 2, tile);
       if (blockActNext && boxNext.fits)
         stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch,
             sImg + ((v + 1) & 1) * kPitch * kFH, wv, laneOff, loHalf);
@@ -570,8 +592,8 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
-            acc[r] += lds_update<PREC, false>(sBase, fmaf(fr, du, u0), fmaf(fr, dv, v0),
-                fmaf(fr, dw, w0), lou, hiu, lov, hiv);
+            acc[r] += lds_update<PREC, false>(sBase, r ? fmaf(fr, du, u0) : u0,
+                r ? fmaf(fr, dv, v0) : v0, r ? fmaf(fr, dw, w0) : w0, lou, hiu, lov, hiv);
             if ((r % LH_FENCE) == LH_FENCE - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
@@ -579,8 +601,8 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
-            acc[r] += lds_update<PREC, true>(sBase, fmaf(fr, du, u0), fmaf(fr, dv, v0),
-                fmaf(fr, dw, w0), lou, hiu, lov, hiv);
+            acc[r] += lds_update<PREC, true>(sBase, r ? fmaf(fr, du, u0) : u0,
+                r ? fmaf(fr, dv, v0) : v0, r ? fmaf(fr, dw, w0) : w0, lou, hiu, lov, hiv);
             if ((r % LH_FENCE) == LH_FENCE - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
@@ -1293,7 +1315,12 @@ static int box_storage(LhContext *c, long nTiles)
   (void)hipFree(c->dBoxes);
   c->dBoxes   = nullptr;
   c->boxTiles = 0;
-  LH_HIP(hipMalloc(&c->dBoxes, (size_t)c->cfg.nSlots * (size_t)nTiles * sizeof(int2)));
+  // the kernels address the boxes through one buffer resource (32-bit range)
+  const uint64_t bytes = (uint64_t)c->cfg.nSlots * (uint64_t)nTiles * sizeof(int2);
+  if (bytes >= (1ull << 32))
+    return fail(LH_E_ARG, "footprint boxes of %u slots x %ld tiles exceed 4 GiB", c->cfg.nSlots,
+        nTiles);
+  LH_HIP(hipMalloc(&c->dBoxes, (size_t)bytes));
   c->boxTiles = nTiles;
   std::fill(c->boxR.begin(), c->boxR.end(), 0);
   return LH_OK;
@@ -1320,7 +1347,8 @@ static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, int
     while (e < first + count && (force || c->boxR[e] != R) && e - s < 65535)
       ++e;
     box_kernel<<<dim3((unsigned)((nTiles + 255) / 256), e - s), 256, 0, c->stream>>>(c->dViews,
-        (int)s, (int)nTiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H, c->dBoxes);
+        (int)s, (int)nTiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H,
+        (int)g.nSlots, c->dBoxes);
     LH_HIP(hipGetLastError());
     for (uint32_t k = s; k < e; ++k)
       c->boxR[k] = R;
@@ -1356,11 +1384,14 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   A.tilesZ      = (int)g.L / (stage ? kTileZ : kWaves);
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
   A.boxes       = nullptr;
+  A.boxSlots    = (int)g.nSlots;
+  A.boxBytes    = 0;
   if (stage) {
     const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, A.tilesZ, forceBoxes);
     if (rc != LH_OK)
       return rc;
-    A.boxes = c->dBoxes;
+    A.boxes    = c->dBoxes;
+    A.boxBytes = (uint32_t)((uint64_t)g.nSlots * (uint64_t)c->boxTiles * sizeof(int2));
   }
 #define LH_LAUNCH(RR, PP, SS) launch_shape<RR, PP, SS>(c, A)
 #define LH_BY_PREC(RR)                                                                   \
