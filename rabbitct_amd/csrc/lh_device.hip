@@ -286,6 +286,8 @@ constexpr int kPitch = LH_PITCH; // LDS row pitch (dwords) >= kFW; ds_read2 offs
 constexpr int kFH    = LH_FH;    // LDS rows per buffer (2 buffers x kFH x kPitch x 4 B <= 40 KB:
                                  // 4 workgroups per CU)
 static_assert(kPitch >= kFW && kPitch + 1 <= 255, "LDS pitch");
+constexpr int kPitchLog2 = kPitch == 128 ? 7 : (kPitch == 256 ? 8 : -1);
+static_assert(kPitchLog2 > 0, "LDS pitch must be a power of two");
 static_assert(2 * kFH * kPitch * 4 <= 40 * 1024, "footprint buffers exceed 40 KB");
 constexpr int kLYs   = 32;  // lanes along y in the staged tile
 constexpr int kTileZ = 8;   // z extent of the staged tile
@@ -443,9 +445,13 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
   } else {
     ax = __builtin_amdgcn_fractf(ix);
     ay = __builtin_amdgcn_fractf(iy);
-    // byte offset row * pitch + 4 * column: one 24-bit multiply-add after the shift
-    q = reinterpret_cast<const float *>(reinterpret_cast<const char *>(sBase) +
-        __mul24((int)iy, kPitch * 4) + ((int)ix << 2));
+    // byte address base + 4 * column + row * 4 * kPitch as two shift-adds: the
+    // empty asm hides the partial sum, which the compiler would otherwise
+    // re-associate into two shifts and a three-way add
+    typedef const __attribute__((address_space(3))) float LdsFloat;
+    uint32_t col = (uint32_t)(uintptr_t)(LdsFloat *)sBase + ((uint32_t)(int)ix << 2);
+    __asm__("" : "+v"(col));
+    q = (const float *)(LdsFloat *)(uintptr_t)(col + ((uint32_t)(int)iy << (kPitchLog2 + 2)));
   }
   float b, t;
   if constexpr (PREC == LH_PREC_F32) {
