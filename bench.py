@@ -90,8 +90,7 @@ def cpu_baseline(args, geom, phantom, ranges):
     runner = orc.ref_runner(isa)
     if runner is None:
         return None
-    n_asm = min(args.cpu_views or geom.n_views, geom.n_views)
-    n_opt = max(4, n_asm // 4)
+    n_asm = n_opt = min(args.cpu_views or geom.n_views, geom.n_views)
     views = np.arange(n_asm)
     with lolahip.Context(args.L, geom.width, geom.height, geom.n_views, n_slots=n_asm) as ctx:
         imgs = generate_into(ctx, geom, phantom, views=views, want_host=True)
@@ -213,7 +212,7 @@ def main():
     ap.add_argument("--variants", default="plain-f32,fused-f32,fused-bf16,fused-f16")
     ap.add_argument("--headline", default="fused-f32")
     ap.add_argument("--cpu-views", type=int, default=0,
-                    help="views of the CPU baseline sample (0: all); LolaOPT runs a quarter")
+                    help="views of the CPU baseline sample (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
