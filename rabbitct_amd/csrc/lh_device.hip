@@ -275,6 +275,11 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
 #ifndef LH_FENCE
 #define LH_FENCE 4 // updates per scheduling group in the staged loop
 #endif
+// updates per scheduling group of the L2-gather fallback, by run length
+// (measured at L=256, where most footprints take it: with R=16 one voxel's
+// loads at a time is fastest, with R=32 four)
+template <int R>
+constexpr int kFenceL2 = R == 32 ? 4 : 1;
 #ifndef LH_PITCH
 #define LH_PITCH 128
 #endif
@@ -633,7 +638,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
           const bool on  = r >= rlo && r < rhi;
           acc[r] += voxel_update<PREC>(rs, A, on ? fmaf(fr, du, u0) : 0.0f,
               on ? fmaf(fr, dv, v0) : 0.0f, on ? fmaf(fr, dw, w0) : kPark);
-          __builtin_amdgcn_sched_barrier(0);
+          // groups of kFenceL2 updates keep that many voxels' tap loads in
+          // flight per lane
+          if ((r % kFenceL2<R>) == kFenceL2<R> - 1)
+            __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
