@@ -583,12 +583,13 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       // pixel from its corner, and the weight (1/w)^2 underflows to +0, so the
       // update adds exactly zero.  The accumulators then take one straight
       // path through every branch below.
-      const float pu = box.fits ? (float)box.iu0 + 0.5f : 0.0f;
-      const float pv = box.fits ? (float)box.iv0 + 0.5f : 0.0f;
+      // (iu0 + 0.5) * 2^100 = (2 iu0 + 1) * 2^99, exact
+      const float pu = box.fits ? ldexpf((float)(2 * box.iu0 + 1), -1) : 0.0f;
+      const float pv = box.fits ? ldexpf((float)(2 * box.iv0 + 1), -1) : 0.0f;
       const bool full = __all(!active || (rlo == 0 && rhi == R));
-      if (full && !active) {
-        u0 = pu * kPark;
-        v0 = pv * kPark;
+      if (full && !__all(active) && !active) {
+        u0 = box.fits ? ldexpf((float)(2 * box.iu0 + 1), 99) : 0.0f;
+        v0 = box.fits ? ldexpf((float)(2 * box.iv0 + 1), 99) : 0.0f;
         w0 = kPark;
       }
       if (box.fits) {
