@@ -107,8 +107,8 @@ struct BpArgs {
   uint32_t boxBytes;       // bytes of the boxes array (< 4 GiB)
 };
 
-// Per-view parameters and footprint boxes are read through the constant
-// address space: uniform addresses then become scalar loads.
+// Per-view parameters are read through the constant address space: uniform
+// addresses then become scalar loads.
 typedef const __attribute__((address_space(4))) ViewParams CViewParams;
 __device__ __forceinline__ CViewParams &view_params(const BpArgs &A, int slot)
 {
@@ -262,16 +262,16 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
 
 // ---- LDS-staged tile ------------------------------------------------------
 // Tile of R (x) x 32 (y) x 8 (z) voxels: lane l of wave w owns
-// (x0..x0+R-1, y0 + l%32, z0 + 2w + l/32).  Per view the workgroup projects
-// the tile's 8 corners, takes the bounding box of their detector positions
-// (plus a 1-pixel margin and the right/lower tap) and copies that footprint of
-// the bordered projection into LDS with direct-to-LDS buffer loads (one
-// instruction moves 64 consecutive pixels of a row, no VGPR round trip).  The
-// footprints are double-buffered: view v+1 streams in while view v computes,
-// one barrier per view.  Voxel coordinates are clamped to the box, so every
-// LDS index is in bounds by construction (for voxels inside the tile the
-// clamp is the identity).  A footprint larger than kFW x kFH falls back to
-// L2 gathers for that view.
+// (x0..x0+R-1, y0 + l%32, z0 + 2w + l/32).  Per (view, tile) box_kernel
+// projects the tile's 8 corners and keeps the bounding box of their detector
+// positions (plus a 1-pixel margin and the right/lower tap); the kernel copies
+// that footprint of the bordered projection into LDS with direct-to-LDS
+// buffer loads (16 bytes per lane, two 128-pixel rows per instruction, no
+// VGPR round trip).  The footprints are double-buffered: view v+1 streams in
+// while view v computes, one barrier per view.  Voxel coordinates are
+// clamped to the box, or, for boxes the detector edge does not clip, lie in
+// it by construction, so every LDS index is in bounds.  A footprint larger
+// than kFW x kFH falls back to L2 gathers for that view.
 #ifndef LH_FENCE
 #define LH_FENCE 4 // updates per scheduling group in the staged loop
 #endif
