@@ -142,6 +142,49 @@ def test_reference_runner_drives_lolahip(tmp_path, golden_border):
     assert pt <= TOL and zs <= TOL
 
 
+def test_reference_runner_full_scan_l512(tmp_path):
+    """The reference runner drives LolaHIP over the whole L=512 workload (496
+    views of 1248x960 from the seeded 10-ellipsoid phantom, -b 16, the clip
+    file computed on the GPU), as the module is used in practice: every view
+    crosses the pinned upload ring inside the timed iterate.  64 sampled lines
+    match the FP64 oracle to 1e-4 of their range.  Prints the runner's Total
+    and its nominal GUPS."""
+    import re
+    runner = orc.ref_runner(orc.host_isa(), lolahip=True) or orc.ref_runner("avx2", lolahip=True)
+    if runner is None:
+        pytest.skip("reference runner with LolaHIP not built")
+    L2, N = 512, 496
+    geom = ScanGeometry(n_views=N, seed=512).build()
+    ph = EllipsoidPhantom(seed=512).build()
+    with lolahip.Context(L2, geom.width, geom.height, N, n_slots=N) as ctx:
+        imgs = generate_into_host(ctx, geom, ph)
+    ranges = lolahip.compute_line_ranges(L2, geom.width, geom.height, geom.matrices, 16,
+                                         device=True)
+    hdr = rct.RctHeader(geom.width, geom.height, N, (1000.0, 0.0))
+    rct.write_rct(tmp_path / "in.rct", hdr, geom.matrices, imgs)
+    rct.write_geometry(tmp_path / "geo.rct", hdr, geom.matrices)
+    rct.write_clip_file(tmp_path / "clip.rct", ranges)
+    r = subprocess.run([runner, "-i", "in.rct", "-a", "geo.rct", "-m", "LolaHIP", "-s", str(L2),
+                        "-b", "16", "-C", "clip.rct", "-o", "hip.vol"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    total = float(re.search(r"Total: ([0-9.eE+-]+) s", r.stdout).group(1))
+    print(f"\nrunner -m LolaHIP -s {L2} -b 16, {N} views: Total {total:.4f} s = "
+          f"{L2 ** 3 * N / total / 1e9:.0f} GUPS (uploads included)")
+    vol = np.memmap(tmp_path / "hip.vol", dtype=np.float32, mode="r", shape=(L2, L2, L2))
+    rng = np.random.default_rng(5)
+    zs = np.concatenate([[0, L2 - 1, L2 // 2, L2 // 2], rng.integers(0, L2, 60)])
+    ys = np.concatenate([[L2 // 2, L2 // 2, 0, L2 - 1], rng.integers(0, L2, 60)])
+    got = np.asarray(vol[zs, ys, :], dtype=np.float64)
+    del vol
+    ref = orc.backproject_lines(L2, voxel_size(L2), volume_origin(L2), geom.matrices, imgs,
+                                ranges, zs, ys)
+    span = float(ref.max() - ref.min())
+    err = float(np.abs(got - ref).max()) / span
+    print(f"sampled lines vs FP64: max |d| {err:.2e} of range")
+    assert err <= TOL
+
+
 def _full_detector_case(L2, n_views, seed, noise):
     geom = ScanGeometry(n_views=n_views, seed=seed).build()
     if noise:
