@@ -471,12 +471,25 @@ __global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict
   rest /= tyN;
   const int tz = rest % tzN, n = rest / tzN;
   const int x0 = tx * 16, y0 = ty * 4, z0 = tz * MT;
-  for (int i = threadIdx.x; i < kHX * kHY * (MT + 2); i += 256) {
-    const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
-    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-    const bool ok = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
-                    (unsigned)gz < (unsigned)D;
-    halo[i] = ok ? f2bf(in[(((size_t)n * D + gz) * D + gy) * D + gx]) : (uint16_t)0;
+  // all of a thread's halo loads are issued before the first is used
+  constexpr int HV = kHX * kHY * (MT + 2), HITER = (HV + 255) / 256;
+  float hv[HITER];
+#pragma unroll
+  for (int k = 0; k < HITER; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    hv[k] = 0.0f;
+    if (i < HV) {
+      const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
+      const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      if ((unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D && (unsigned)gz < (unsigned)D)
+        hv[k] = in[(((size_t)n * D + gz) * D + gy) * D + gx];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < HITER; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    if (i < HV)
+      halo[i] = f2bf(hv[k]);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1126,7 +1139,7 @@ ConvIn dec_in(LhUnet *u, int lev)
 // concatenation is materialised).  Inference fuses the head into the last
 // convolution; `training` keeps every activation the backward pass reads
 // (and the padded first-layer input for its weight gradient).
-int forward(LhUnet *u, const float *in, int n, int D, bool training)
+int forward(LhUnet *u, const float *in, int n, int D, bool training, float *outp = nullptr)
 {
   if (D % 64)
     return lh_fail(LH_E_ARG, "U-Net input edge must be a multiple of 64 (got %d)", D);
@@ -1174,7 +1187,7 @@ int forward(LhUnet *u, const float *in, int n, int D, bool training)
     h.in = in;
     h.w = u->hw;
     h.b = u->hb;
-    h.out = u->out;
+    h.out = outp ? outp : u->out;
     Act none;
     none.p = nullptr;
     none.ld = 16;
@@ -1423,11 +1436,11 @@ int lhUnetForward(LhUnet *u, const float *dIn, float *dOut, int n, int D)
 {
   if (!u || !dIn || !dOut || n < 1)
     return lh_fail(LH_E_ARG, "lhUnetForward: bad arguments");
-  int rc = forward(u, dIn, n, D, false);
+  // the fused head writes the caller's buffer (each voxel reads its input
+  // before writing its output, so dOut may be dIn)
+  int rc = forward(u, dIn, n, D, false, dOut);
   if (rc != LH_OK)
     return rc;
-  const size_t nv = (size_t)n * D * D * D;
-  UN_HIP(hipMemcpyAsync(dOut, u->out, nv * sizeof(float), hipMemcpyDeviceToDevice, u->stream));
   UN_HIP(hipStreamSynchronize(u->stream));
   return LH_OK;
 }
