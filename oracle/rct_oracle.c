@@ -6,13 +6,15 @@
  *
  * Plain C, FP64 throughout.  Each function names the reference file:line it
  * restates (paths relative to the RabbitCT source tree).  Pinning (the checks
- * that make these restatements trustworthy) lives in tests/test_oracle_*.py:
+ * that make these restatements trustworthy) lives in tests/test_oracle.py:
  *   orc_backproject_bunny      golden vectors of the compiled LolaBunny, closed
  *                              forms (constant image, single pixel, 1/s^2 scaling)
  *   orc_backproject_clipped    golden vectors of the compiled LolaOPT/LolaOMP,
  *                              agreement with bunny inside the clip ranges
  *   orc_backproject_lines      equality with orc_backproject_clipped on the
  *                              same lines (it is that loop on a line subset)
+ *   orc_backproject_opt_f32    golden vectors of the compiled LolaOPT (within
+ *                              5e-7 of the range, closer than the FP64 form)
  *   orc_backproject_asm        golden vectors of the compiled LolaASM (the
  *                              do-while "at least one vector" semantics)
  *   orc_line_ranges            byte equality with clip files written by the
