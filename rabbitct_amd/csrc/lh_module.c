@@ -8,7 +8,10 @@
  *
  * Tunables (environment; the runner's command line stays unchanged):
  *   LOLAHIP_PRECISION       f32 (default) | bf16 | f16   projection storage
- *   LOLAHIP_VIEWS_PER_PASS  views per register pass (default 32; 1 = plain)
+ *   LOLAHIP_VIEWS_PER_PASS  views per register pass (default 16; 1 = plain)
+ *   LOLAHIP_GRAPHS          1: replay each batch as a captured HIP graph
+ *                           (default 0: in one runner process the capture
+ *                           costs more than the replays save)
  *   LOLAHIP_RUN             voxels per thread along x: 8 | 16 (default) | 32
  *   LOLAHIP_VECTORSIZE      clip alignment for computed ranges: 4 | 8 | 16 (default)
  *   LOLAHIP_CLIP_DEVICE     gpu (default) | cpu: where missing ranges are computed
@@ -233,7 +236,7 @@ int lolaHipPrepare(RabbitCtGlobalData *rcgd)
   }
   const uint32_t L = rcgd->problemSize, N = rcgd->numberOfProjections;
   const int vs     = env_int("LOLAHIP_VECTORSIZE", 16);
-  int per          = env_int("LOLAHIP_VIEWS_PER_PASS", 32);
+  int per          = env_int("LOLAHIP_VIEWS_PER_PASS", 16);
   if (per < 1)
     per = 1;
   if ((uint32_t)per > N)
@@ -254,8 +257,8 @@ int lolaHipPrepare(RabbitCtGlobalData *rcgd)
   int rc = lhCreate(&cfg, &Ctx);
   if (rc != LH_OK)
     die("device set-up", rc);
-  /* batches replay captured HIP graphs unless LOLAHIP_GRAPHS=0 */
-  if ((rc = lhSetGraphs(Ctx, env_int("LOLAHIP_GRAPHS", 1))) != LH_OK)
+  /* batches replay captured HIP graphs with LOLAHIP_GRAPHS=1 */
+  if ((rc = lhSetGraphs(Ctx, env_int("LOLAHIP_GRAPHS", 0))) != LH_OK)
     die("graph set-up", rc);
 
   LhLineRange *range = (LhLineRange *)malloc((size_t)N * L * L * sizeof(LhLineRange));
