@@ -159,6 +159,51 @@ __device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, 
   }
 }
 
+// Coarse form for the upsampled channels [cbase, cbase + 32) of a decoder
+// convolution (a.s0 at half resolution): the fine halo [x0-1, x0+16] x
+// [y0-1, y0+4] x [z0-1, z0+MT] reads coarse voxels [x0/2-1, x0/2+8] x
+// [y0/2-1, y0/2+2] x [z0/2-1, z0/2+MT/2], staged once as
+// [MT/2 + 2][kCHY][kCHX][32] (zeros outside the coarse volume, which is where
+// the fine halo leaves the volume) instead of 8 replicated copies.
+#ifndef LH_UPC
+#define LH_UPC 1
+#endif
+constexpr int kCHX = 10, kCHY = 4;
+template <int MT>
+__device__ __forceinline__ void stage_coarse(const ConvLdsArgs &a, uint16_t *lds, int cbase, int n,
+    int x0, int y0, int z0)
+{
+  constexpr int CHZ   = MT / 2 + 2;
+  constexpr int TOTAL = kCHX * kCHY * CHZ * 4;  // 16-byte units (32 channels per voxel)
+  constexpr int ITER  = (TOTAL + 255) / 256;
+  const int Dc = a.D >> 1;
+  const ConvSrc &sr = a.s0;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+  u32x4 val[ITER];
+#pragma unroll
+  for (int k = 0; k < ITER; ++k) {
+    const int i = threadIdx.x + k * 256;
+    if (i < TOTAL) {
+      const int q = i % 4, v = i / 4;
+      const int cx = (x0 >> 1) - 1 + v % kCHX, cy = (y0 >> 1) - 1 + (v / kCHX) % kCHY,
+                cz = (z0 >> 1) - 1 + v / (kCHX * kCHY);
+      const bool in = (unsigned)cx < (unsigned)Dc && (unsigned)cy < (unsigned)Dc &&
+                      (unsigned)cz < (unsigned)Dc;
+      const uint32_t off = in ? (uint32_t)(((((uint32_t)n * Dc + cz) * Dc + cy) * Dc + cx) *
+                                           (uint32_t)sr.ld + sr.ofs + cbase + 8 * q) * 2u
+                              : kMissOffset;
+      val[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ITER; ++k) {
+    const int i = threadIdx.x + k * 256;
+    if (i < TOTAL)
+      *reinterpret_cast<u32x4 *>(lds + (i / 4) * 32 + 8 * (i % 4)) = val[k];
+  }
+}
+
 template <int NT>
 __device__ __forceinline__ void load_b(bf16x8 (&B)[NT], const uint16_t *wrow, size_t wtile,
     int chunk)
@@ -180,7 +225,7 @@ template <int NT, int MT> constexpr size_t conv_lds_bytes()
 // Input channels are staged in passes: F = cin/32 passes of 32 channels
 // (one tap per K chunk), then with TAIL a pass of the last 16 channels (tap
 // pairs per K chunk), matching the weight packing.
-template <int NT, int MT, bool TAIL>
+template <int NT, int MT, bool TAIL, bool UPC = false>
 __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
 {
   extern __shared__ __align__(16) uint16_t lds[];
@@ -210,8 +255,51 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
   for (int pass = 0; pass < F; ++pass) {
     if (pass)
       __syncthreads();
-    stage_halo<32, MT>(a, lds, 32 * pass, n, x0, y0, z0);
+    // UPC: every 32-channel pass holds upsampled channels, staged coarse
+    if constexpr (UPC)
+      stage_coarse<MT>(a, lds, 32 * pass, n, x0, y0, z0);
+    else
+      stage_halo<32, MT>(a, lds, 32 * pass, n, x0, y0, z0);
     __syncthreads();
+    if constexpr (UPC) {
+      static_assert(NT <= 2, "coarse staging is built for the plane-reuse layout");
+      // as below, with fine halo voxel (h) read at coarse halo voxel
+      // ((h - 1) >> 1) + 1 per axis; pairs of lanes read the same voxel
+      constexpr int CRS = kCHX * kCHY * 32;  // coarse z-plane stride
+      bf16x8 Bn[3][NT];
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz)
+        load_b(Bn[dz], wrow, wtile, (9 * dz) * F + pass);
+      for (int dyx = 0; dyx < 9; ++dyx) {
+        bf16x8 B[3][NT];
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            B[dz][t] = Bn[dz][t];
+        if (dyx + 1 < 9)
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz)
+            load_b(Bn[dz], wrow, wtile, (9 * dz + dyx + 1) * F + pass);
+        const int dy = dyx / 3, dx = dyx % 3;
+        const int cx = ((m + dx - 1) >> 1) + 1, cy = ((w + dy - 1) >> 1) + 1;
+        const uint16_t *base = lds + (cy * kCHX + cx) * 32 + 8 * kg;
+#pragma unroll
+        for (int p = 0; p < MT + 2; ++p) {
+          const int cz = ((p - 1) >> 1) + 1;
+          const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + cz * CRS));
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz) {
+            const int r = p - dz;
+            if (r >= 0 && r < MT)
+#pragma unroll
+              for (int t = 0; t < NT; ++t)
+                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[dz][t], acc[r][t], 0, 0, 0);
+          }
+        }
+      }
+      continue;
+    }
     if constexpr (NT <= 2) {
       // Input planes outer: the A operand of halo plane p at (dy, dx) serves
       // output rows p - dz for dz = 0, 1, 2 (taps 9 dz + 3 dy + dx), so each
@@ -914,29 +1002,37 @@ ConvIn plain_in(Act a)
   return c;
 }
 
-template <int NT, int MT, bool TAIL> int launch_lds(hipStream_t s, ConvLdsArgs &a)
+template <int NT, int MT, bool TAIL, bool UPC = false> int launch_lds(hipStream_t s, ConvLdsArgs &a)
 {
   constexpr size_t ldsMax = conv_lds_bytes<NT, MT>();
   static bool attr = false;
   if (!attr) {
-    UN_HIP(hipFuncSetAttribute((const void *)conv3_lds_kernel<NT, MT, TAIL>,
+    UN_HIP(hipFuncSetAttribute((const void *)conv3_lds_kernel<NT, MT, TAIL, UPC>,
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsMax));
     attr = true;
   }
-  // a 16-channel input stages a 16-channel halo only
+  // a 16-channel input stages a 16-channel halo only; with every 32-channel
+  // pass staged coarse, the coarse halo or the 16-channel tail sizes it
   const size_t halo16 = (size_t)(MT + 2) * kHY * kHX * 16 * 2;
   const size_t outb = (size_t)4 * MT * 16 * NT * 16 * 2;
-  const size_t lds = (a.cin >= 32) ? ldsMax : std::max(halo16, outb);
+  const size_t coarse = (size_t)(MT / 2 + 2) * kCHY * kCHX * 32 * 2;
+  size_t lds = (a.cin >= 32) ? ldsMax : std::max(halo16, outb);
+  if (UPC)
+    lds = std::max(std::max(coarse, TAIL ? halo16 : 0), outb);
   if (a.D % MT)
     return lh_fail(LH_E_ARG, "conv: D=%d is not a multiple of the z tile %d", a.D, MT);
   const size_t tiles = (size_t)a.n * (a.D / 16) * (a.D / 4) * (a.D / MT);
   a.tilesPerXcd = (tiles % 8 == 0) ? (int)(tiles / 8) : 0;
-  conv3_lds_kernel<NT, MT, TAIL><<<(unsigned)tiles, 256, lds, s>>>(a);
+  conv3_lds_kernel<NT, MT, TAIL, UPC><<<(unsigned)tiles, 256, lds, s>>>(a);
   return LH_OK;
 }
 
-template <bool TAIL> int launch_cout(hipStream_t s, ConvLdsArgs &a, int cout)
+template <bool TAIL> int launch_cout(hipStream_t s, ConvLdsArgs &a, int cout, bool upc)
 {
+  if (upc && cout == 16)
+    return launch_lds<1, 8, TAIL, true>(s, a);
+  if (upc && cout == 32)
+    return launch_lds<2, 8, TAIL, true>(s, a);
   switch (cout) {
   case 16: return launch_lds<1, 8, TAIL>(s, a);
   case 32: return launch_lds<2, 8, TAIL>(s, a);
@@ -1018,7 +1114,10 @@ int conv_fwd(hipStream_t s, const Conv &L, ConvIn in, Act out, int D, int n, boo
   if (a.cin > 96 || (out.p && ((out.ld % 8) || (out.ofs % 8))))
     return lh_fail(LH_E_ARG, "conv: unsupported channels (in %d, out stride %d offset %d)", a.cin,
         out.ld, out.ofs);
-  rc = (a.cin % 32) ? launch_cout<true>(s, a, cout) : launch_cout<false>(s, a, cout);
+  // Upsampled channels (a half-resolution a.s0 filling every 32-channel
+  // pass) are staged once at coarse resolution; plane-reuse layouts only.
+  const bool upc = in.half0 && a.c0 == 32 * (a.cin / 32) && (cout == 16 || cout == 32) && LH_UPC;
+  rc = (a.cin % 32) ? launch_cout<true>(s, a, cout, upc) : launch_cout<false>(s, a, cout, upc);
   if (rc != LH_OK)
     return rc;
   UN_HIP(hipGetLastError());
