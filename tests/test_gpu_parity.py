@@ -327,7 +327,8 @@ def test_l1024_sampled_lines_vs_fp64(kernel, prec):
     oracle.  FP32 storage: max |d| <= 2e-4 of the sampled range (8 views,
     LolaOPT-level FP32 rounding); FP16 storage: RMSE <= 1e-3.  Drift: the
     mean error over the last quarter of each line is within 1.5x of the first
-    quarter's (the per-run FP64 origin keeps x-stepping from accumulating)."""
+    quarter's (each run of voxels restarts from its own origin, so the
+    x-stepping does not accumulate)."""
     L2, nv = 1024, 8
     geom = ScanGeometry(n_views=496, seed=1024).build().subset(np.arange(0, 496, 62))
     ph = EllipsoidPhantom(n_ell=30, seed=1024).build()
