@@ -116,7 +116,20 @@ __device__ __forceinline__ CViewParams &view_params(const BpArgs &A, int slot)
   return ((CViewParams *)(uintptr_t)A.views)[slot];
 }
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const BpArgs &A, int slot)
+// The kernel's argument block read again through the kernarg segment pointer,
+// which an empty asm hides from the compiler: fields used through it in a
+// loop are then reloaded with scalar loads where they are needed instead of
+// being held live (and spilled to VGPR lanes) across the whole loop.
+typedef const __attribute__((address_space(4))) BpArgs KernelArgs;
+__device__ __forceinline__ KernelArgs &kernel_args()
+{
+  KernelArgs *k = (KernelArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(k));
+  return *k;
+}
+
+template <class Args>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(Args &A, int slot)
 {
   return __builtin_amdgcn_make_buffer_rsrc(
       (void *)(A.slots + (uint64_t)slot * A.slotBytes), (short)0, (int)A.slotRecords,
@@ -139,8 +152,8 @@ __device__ __forceinline__ float f16_hi(uint32_t q)
 //  FP32: row-major padded image, four 4-byte loads (rows iiy and iiy+1).
 //  BF16/F16: "pair" layout, element (i,j) holds p(i,j), p(i+1,j) as 2 halves
 //  in one dword: two 4-byte loads (rows iiy and iiy+1).
-template <int PREC>
-__device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, const BpArgs &A,
+template <int PREC, class Args>
+__device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, Args &A,
     int iix, int iiy, float &p00, float &p10, float &p01, float &p11)
 {
   if constexpr (PREC == LH_PREC_F32) {
@@ -165,8 +178,8 @@ __device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, const BpAr
 
 // One voxel-view update: returns val / w^2 for the voxel at homogeneous
 // (u, v, w) (LolaOPT.c:145-170).
-template <int PREC>
-__device__ __forceinline__ float voxel_update(__amdgpu_buffer_rsrc_t rs, const BpArgs &A,
+template <int PREC, class Args>
+__device__ __forceinline__ float voxel_update(__amdgpu_buffer_rsrc_t rs, Args &A,
     float u, float v, float w)
 {
   const float rw = __builtin_amdgcn_rcpf(w);
@@ -313,13 +326,14 @@ struct StageBox {
 // consecutive views share cache lines, and are read with vector loads issued
 // two views ahead (they land under the same vmcnt wait as the footprints):
 // a scalar load would be waited for by every LDS access (lgkmcnt).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const BpArgs &A)
+template <class Args> __device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(Args &A)
 {
   return __builtin_amdgcn_make_buffer_rsrc((void *)A.boxes, (short)0, (int)A.boxBytes,
       (int)kRsrcWord3);
 }
 
-__device__ __forceinline__ uint2 fetch_box(__amdgpu_buffer_rsrc_t rs, const BpArgs &A, int slot,
+template <class Args>
+__device__ __forceinline__ uint2 fetch_box(__amdgpu_buffer_rsrc_t rs, Args &A, int slot,
     int tile)
 {
   const uint32_t off = ((uint32_t)tile * (uint32_t)A.boxSlots + (uint32_t)slot) * 8u;
@@ -563,10 +577,11 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     StageBox boxNext = box;
     if (hasNext) {
       boxNext = decode_box(boxRaw);
+      KernelArgs &K = kernel_args();
       if (v + 2 < n)
-        boxRaw = fetch_box(boxRs, A, slot + 2, tile);
+        boxRaw = fetch_box(box_rsrc(K), K, slot + 2, tile);
       if (blockActNext && boxNext.fits)
-        stage_issue<ELEM>(slot_rsrc(A, slot + 1), boxNext, pitch,
+        stage_issue<ELEM>(slot_rsrc(K, slot + 1), boxNext, pitch,
             sImg + ((v + 1) & 1) * kPitch * kFH, wv, laneOff, loHalf);
     }
 
@@ -637,12 +652,13 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         }
       } else {
         // footprint larger than the LDS window: L2 gathers, one voxel at a time
-        const __amdgpu_buffer_rsrc_t rs = slot_rsrc(A, slot);
+        KernelArgs &K = kernel_args();
+        const __amdgpu_buffer_rsrc_t rs = slot_rsrc(K, slot);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const float fr = (float)r;
           const bool on  = r >= rlo && r < rhi;
-          acc[r] += voxel_update<PREC>(rs, A, on ? fmaf(fr, du, u0) : 0.0f,
+          acc[r] += voxel_update<PREC>(rs, K, on ? fmaf(fr, du, u0) : 0.0f,
               on ? fmaf(fr, dv, v0) : 0.0f, on ? fmaf(fr, dw, w0) : kPark);
           // groups of kFenceL2 updates keep that many voxels' tap loads in
           // flight per lane
