@@ -592,8 +592,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     const bool active = rlo < rhi;
     if (blockAct && __any(active)) {
       touched |= active;
-      // run origin in FP32 (as the reference's own loops compute it): nine
-      // FMAs on scalar operands instead of FP64 arithmetic and conversions
+      // run origin in FP32, as the reference's ISPC kernel evaluates its
+      // coordinates (fastRabbit.ispc:23-26; LolaOPT.c:139-147 rounds FP64
+      // partial sums to float): nine FMAs on scalar operands instead of FP64
+      // arithmetic and conversions
       float u0 = fmaf(P.af[0], fx0, fmaf(P.af[3], fy, fmaf(P.af[6], fz, P.af[9])));
       float v0 = fmaf(P.af[1], fx0, fmaf(P.af[4], fy, fmaf(P.af[7], fz, P.af[10])));
       float w0 = fmaf(P.af[2], fx0, fmaf(P.af[5], fy, fmaf(P.af[8], fz, P.af[11])));
