@@ -287,9 +287,12 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
 // clamped to the box, or, for boxes the detector edge does not clip, lie in
 // it by construction, so every LDS index is in bounds.  A footprint larger
 // than kFW x kFH falls back to L2 gathers for that view.
-#ifndef LH_FENCE
-#define LH_FENCE 4 // updates per scheduling group in the staged loop
-#endif
+// updates per scheduling group in the staged loop, by precision (measured
+// at L=512 with R=16: FP32 fastest with groups of 16, that is one fence at
+// the end of the run, 45.7 against 47.2 ms per step with groups of 4; the
+// half-precision kernels fastest with 4)
+template <int PREC>
+constexpr int kFence = PREC == LH_PREC_F32 ? 16 : 4;
 // updates per scheduling group of the L2-gather fallback, by run length
 // (measured at L=256, where most footprints take it: with R=16 one voxel's
 // loads at a time is fastest, with R=32 four)
@@ -619,7 +622,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         const float *sBase = sImg + (v & 1) * kPitch * kFH - (box.iv0 * kPitch + box.iu0);
         const float lou = (float)box.iu0, hiu = (float)(box.iu0 + box.fw - 2);
         const float lov = (float)box.iv0, hiv = (float)(box.iv0 + box.fh - 2);
-        // groups of LH_FENCE updates are fenced so the scheduler keeps at most
+        // groups of kFence updates are fenced so the scheduler keeps at most
         // that many updates' taps in flight per lane
         if (full && box.interior) {
           // with VECTORSIZE-aligned ranges a run is all in or all out
@@ -628,7 +631,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
             const float fr = (float)r;
             acc[r] += lds_update<PREC, false>(sBase, r ? fmaf(fr, du, u0) : u0,
                 r ? fmaf(fr, dv, v0) : v0, r ? fmaf(fr, dw, w0) : w0, lou, hiu, lov, hiv);
-            if ((r % LH_FENCE) == LH_FENCE - 1)
+            if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         } else if (full) {
@@ -637,7 +640,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
             const float fr = (float)r;
             acc[r] += lds_update<PREC, true>(sBase, r ? fmaf(fr, du, u0) : u0,
                 r ? fmaf(fr, dv, v0) : v0, r ? fmaf(fr, dw, w0) : w0, lou, hiu, lov, hiv);
-            if ((r % LH_FENCE) == LH_FENCE - 1)
+            if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         } else {
@@ -648,7 +651,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
             acc[r] += lds_update<PREC, true>(sBase, on ? fmaf(fr, du, u0) : pu * kPark,
                 on ? fmaf(fr, dv, v0) : pv * kPark, on ? fmaf(fr, dw, w0) : kPark, lou, hiu,
                 lov, hiv);
-            if ((r % LH_FENCE) == LH_FENCE - 1)
+            if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         }
