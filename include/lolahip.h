@@ -293,6 +293,11 @@ int lhConv3dTest(const uint16_t *dx, int cin, int cout, const float *W, const fl
     uint16_t *dy, int n, int D, int relu, int backward);
 int lhConv3dWgradTest(const uint16_t *dx, int cin, const uint16_t *dg, int cout, float *dW,
     int n, int D);
+/* Decoder form (tests): channels [0, c0) from dx0 ([n*Ds^3][c0], Ds = D/2 when
+ * half0 = 1, nearest upsampling read in place), channels [c0, cin) from dx1
+ * ([n*D^3][cin-c0]); dy receives [n*D^3][cout]. */
+int lhConv3dSplitTest(const uint16_t *dx0, int c0, int half0, const uint16_t *dx1, int cin,
+    int cout, const float *W, const float *b, uint16_t *dy, int n, int D, int relu);
 
 /* Device memory for callers without another allocator. */
 int lhDeviceAlloc(void **p, uint64_t bytes);

@@ -694,6 +694,8 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 // chip walks the tiles; tile t goes to XCD (t mod 8) in dispatch order, so the
 // tiles are renumbered such that each XCD's share is one contiguous z-slab and
 // its L2 serves one band of detector rows.
+// A must stay the kernel's only argument: kernel_args() reads it back from
+// offset 0 of the kernarg segment.
 template <int R, int PREC, int STAGE>
 __global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpArgs A)
 {

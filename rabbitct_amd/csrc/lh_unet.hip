@@ -1388,6 +1388,38 @@ int adam_all(LhUnet *u, float lr)
 
 } // namespace
 
+namespace {
+// The layer of a test convolution: FP32 weights uploaded and packed; every
+// device buffer is released on destruction, also after a failed step.
+struct TestConv {
+  Conv L;
+  ~TestConv()
+  {
+    for (void *p : { (void *)L.W, (void *)L.b, (void *)L.wf, (void *)L.wb })
+      if (p)
+        (void)hipFree(p);
+  }
+  int init(int cin, int cout, const float *W, const float *b)
+  {
+    L.cin = cin;
+    L.cout = cout;
+    L.kpf = 32 * kchunks(cin);
+    L.kpb = 32 * kchunks(cout);
+    const size_t nw = (size_t)cout * 27 * cin;
+    int rc;
+    if ((rc = dalloc(&L.W, nw)) || (rc = dalloc(&L.b, cout)) ||
+        (rc = dalloc(&L.wf, (size_t)cout * L.kpf)) || (rc = dalloc(&L.wb, (size_t)cin * L.kpb)))
+      return rc;
+    UN_HIP(hipMemcpy(L.W, W, nw * sizeof(float), hipMemcpyHostToDevice));
+    if (b)
+      UN_HIP(hipMemcpy(L.b, b, cout * sizeof(float), hipMemcpyHostToDevice));
+    pack_weights_kernel<<<grid_for(nw), 256>>>(L.W, cin, cout, L.wf, L.kpf, L.wb, L.kpb);
+    UN_HIP(hipGetLastError());
+    return LH_OK;
+  }
+};
+} // namespace
+
 extern "C" {
 
 int lhUnetCreate(LhUnet **out, uint64_t seed)
@@ -1576,28 +1608,40 @@ int lhUnetTrainStep(LhUnet *u, const float *dIn, const float *dTarget, int n, in
 int lhConv3dTest(const uint16_t *dx, int cin, int cout, const float *W, const float *b,
     uint16_t *dy, int n, int D, int relu, int backward)
 {
-  Conv L;
-  L.cin = cin;
-  L.cout = cout;
-  L.kpf = 32 * kchunks(cin);
-  L.kpb = 32 * kchunks(cout);
-  const size_t nw = (size_t)cout * 27 * cin;
-  int rc;
-  if ((rc = dalloc(&L.W, nw)) || (rc = dalloc(&L.b, cout)) || (rc = dalloc(&L.wf, (size_t)cout * L.kpf)) ||
-      (rc = dalloc(&L.wb, (size_t)cin * L.kpb)))
+  TestConv T;
+  int rc = T.init(cin, cout, W, b);
+  if (rc != LH_OK)
     return rc;
-  UN_HIP(hipMemcpy(L.W, W, nw * sizeof(float), hipMemcpyHostToDevice));
-  if (b)
-    UN_HIP(hipMemcpy(L.b, b, cout * sizeof(float), hipMemcpyHostToDevice));
-  pack_weights_kernel<<<grid_for(nw), 256>>>(L.W, cin, cout, L.wf, L.kpf, L.wb, L.kpb);
   Act in, out;
   in.p = (uint16_t *)dx; in.ld = backward ? cout : cin; in.ofs = 0;
   out.p = dy; out.ld = backward ? cin : cout; out.ofs = 0;
-  rc = conv_fwd(0, L, in, out, D, n, relu && !backward, b != nullptr && !backward,
+  rc = conv_fwd(0, T.L, in, out, D, n, relu && !backward, b != nullptr && !backward,
       false, backward != 0);
   UN_HIP(hipDeviceSynchronize());
-  for (void *p : { (void *)L.W, (void *)L.b, (void *)L.wf, (void *)L.wb })
-    (void)hipFree(p);
+  return rc;
+}
+
+// Decoder-form convolution entry for tests: input channels [0, c0) read from
+// dx0 ([n*Ds^3][c0] bf16, Ds = D/2 with half0 = 1: nearest upsampling read in
+// place), channels [c0, cin) from dx1 ([n*D^3][cin-c0] bf16); y [n*D^3][cout].
+int lhConv3dSplitTest(const uint16_t *dx0, int c0, int half0, const uint16_t *dx1, int cin,
+    int cout, const float *W, const float *b, uint16_t *dy, int n, int D, int relu)
+{
+  if (c0 <= 0 || c0 >= cin)
+    return lh_fail(LH_E_ARG, "conv split test: need 0 < c0 < cin (got %d, %d)", c0, cin);
+  TestConv T;
+  int rc = T.init(cin, cout, W, b);
+  if (rc != LH_OK)
+    return rc;
+  ConvIn in;
+  in.a0.p = (uint16_t *)dx0; in.a0.ld = c0; in.a0.ofs = 0;
+  in.half0 = half0 ? 1 : 0;
+  in.c0 = c0;
+  in.a1.p = (uint16_t *)dx1; in.a1.ld = cin - c0; in.a1.ofs = 0;
+  Act out;
+  out.p = dy; out.ld = cout; out.ofs = 0;
+  rc = conv_fwd(0, T.L, in, out, D, n, relu != 0, b != nullptr, false, false);
+  UN_HIP(hipDeviceSynchronize());
   return rc;
 }
 

@@ -79,6 +79,7 @@ EXPORTS = (
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
     "lhUnetGetGrads", "lhUnetForward", "lhUnetTrainStep", "lhConv3dTest", "lhConv3dWgradTest",
+    "lhConv3dSplitTest",
     "lhDeviceAlloc", "lhDeviceFree", "lhCopyToDevice", "lhCopyToHost",
     # before and after the path
     "lhFdkFilter", "lhHuMetric", "lhHuConvert", "lhGetConfig", "lhLoadRctFile",
@@ -155,6 +156,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhUnetTrainStep.argtypes = [P, P, P, i32, i32, f32, ctypes.POINTER(f32)]
     lib.lhConv3dTest.argtypes = [P, i32, i32, P, P, P, i32, i32, i32, i32]
     lib.lhConv3dWgradTest.argtypes = [P, i32, P, i32, P, i32, i32]
+    lib.lhConv3dSplitTest.argtypes = [P, i32, i32, P, i32, i32, P, P, P, i32, i32, i32]
     lib.lhDeviceAlloc.argtypes = [ctypes.POINTER(P), u64]
     lib.lhDeviceFree.argtypes = [P]
     lib.lhCopyToDevice.argtypes = [P, P, u64]

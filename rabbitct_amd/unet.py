@@ -200,6 +200,29 @@ def conv3d(x_bits: np.ndarray, W, b=None, relu=False, backward=False) -> np.ndar
         return dy.download(out_shape, np.uint16)
 
 
+def conv3d_split(x0_bits: np.ndarray, x1_bits: np.ndarray, W, b=None, relu=False,
+                 half0=True) -> np.ndarray:
+    """Decoder-form convolution on the GPU (test entry): input channels
+    [0, c0) from x0_bits (n,Ds,Ds,Ds,c0) bf16 bits, Ds = D/2 with half0 (the
+    nearest upsampling is read in place), the rest from x1_bits
+    (n,D,D,D,cin-c0); W float32 [cout][27][cin].  Returns (n,D,D,D,cout)
+    bf16 bits."""
+    lib = lolahip.load_library()
+    x0_bits = np.ascontiguousarray(x0_bits, dtype=np.uint16)
+    x1_bits = np.ascontiguousarray(x1_bits, dtype=np.uint16)
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    cout, _, cin = W.shape
+    n, D, c0 = x1_bits.shape[0], x1_bits.shape[1], x0_bits.shape[-1]
+    out_shape = (n, D, D, D, cout)
+    bb = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
+    with DeviceBuffer.from_array(x0_bits) as d0, DeviceBuffer.from_array(x1_bits) as d1, \
+            DeviceBuffer(int(np.prod(out_shape)) * 2) as dy:
+        _check(lib.lhConv3dSplitTest(d0.ptr, c0, 1 if half0 else 0, d1.ptr, cin, cout, _p(W),
+                                     None if bb is None else _p(bb), dy.ptr, n, D,
+                                     1 if relu else 0), "lhConv3dSplitTest")
+        return dy.download(out_shape, np.uint16)
+
+
 def conv3d_wgrad(x_bits: np.ndarray, g_bits: np.ndarray) -> np.ndarray:
     """Weight gradient [cout][27][cin] (float32) of one convolution on the GPU."""
     lib = lolahip.load_library()

@@ -41,6 +41,29 @@ def test_conv_forward_matches_oracle(cin, cout, D):
     assert np.all(err <= 2.0 ** -8 * np.abs(ref) + 1e-6 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("c0,c1,cout", [(32, 16, 16), (32, 16, 32), (64, 32, 32), (32, 32, 32)])
+def test_conv_decoder_form_matches_oracle(c0, c1, cout):
+    """Decoder convolutions read their input as [up2(x0), x1] without
+    materialising it: channels [0, c0) at half resolution, the skip channels
+    after them.  (32, 16, *) is dec1a's shape, whose upsampled channels are
+    staged at coarse resolution; the others take the fine halo.  The result
+    must equal the oracle's convolution of the explicit upsampling and
+    concatenation to one BF16 rounding."""
+    rng = np.random.default_rng(c0 * 1000 + c1 * 10 + cout)
+    D = 32
+    x0 = _bf16_normal(rng, (2, D // 2, D // 2, D // 2, c0))
+    x1 = _bf16_normal(rng, (2, D, D, D, c1))
+    W = _weights(rng, c0 + c1, cout)
+    b = rng.normal(size=cout).astype(np.float32)
+    got = uo.from_bf16_bits(unet.conv3d_split(uo.to_bf16_bits(x0), uo.to_bf16_bits(x1), W, b,
+                                              relu=True))
+    x = np.concatenate([uo.up2(x0), x1], axis=-1)
+    ref = np.maximum(uo.conv3d(x, W, b), 0.0)
+    err = np.abs(got - ref)
+    print(f"\ndecoder conv ({c0} up + {c1})->{cout}: max rel {np.max(err / (np.abs(ref) + 1e-30)):.2e}")
+    assert np.all(err <= 2.0 ** -8 * np.abs(ref) + 1e-6 * np.abs(ref).max())
+
+
 @pytest.mark.parametrize("cin,cout", CONV_CASES)
 def test_conv_backward_data_matches_oracle(cin, cout):
     rng = np.random.default_rng(cin * 7 + cout)
