@@ -1395,7 +1395,8 @@ struct TestConv {
   Conv L;
   ~TestConv()
   {
-    for (void *p : { (void *)L.W, (void *)L.b, (void *)L.wf, (void *)L.wb })
+    for (void *p : { (void *)L.W, (void *)L.b, (void *)L.wf, (void *)L.wb, (void *)L.gW,
+             (void *)L.gb })
       if (p)
         (void)hipFree(p);
   }
@@ -1649,7 +1650,8 @@ int lhConv3dSplitTest(const uint16_t *dx0, int c0, int half0, const uint16_t *dx
 int lhConv3dWgradTest(const uint16_t *dx, int cin, const uint16_t *dg, int cout, float *dW,
     int n, int D)
 {
-  Conv L;
+  TestConv T;
+  Conv &L = T.L;
   L.cin = cin;
   L.cout = cout;
   const size_t nw = (size_t)cout * 27 * cin;
@@ -1659,12 +1661,11 @@ int lhConv3dWgradTest(const uint16_t *dx, int cin, const uint16_t *dg, int cout,
   Act x, g;
   x.p = (uint16_t *)dx; x.ld = cin; x.ofs = 0;
   g.p = (uint16_t *)dg; g.ld = cout; g.ofs = 0;
-  rc = conv_wgrad(0, L, x, g, D, n);
+  if ((rc = conv_wgrad(0, L, x, g, D, n)) != LH_OK)
+    return rc;
   UN_HIP(hipDeviceSynchronize());
   UN_HIP(hipMemcpy(dW, L.gW, nw * sizeof(float), hipMemcpyDeviceToHost));
-  (void)hipFree(L.gW);
-  (void)hipFree(L.gb);
-  return rc;
+  return LH_OK;
 }
 
 // Device memory helpers for callers without another allocator.
