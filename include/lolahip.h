@@ -148,6 +148,8 @@ typedef struct LhContext LhContext;
 /* Fills cfg with defaults for (L, W, H, nViews). */
 void lhDefaultConfig(LhConfig *cfg, uint32_t L, uint32_t W, uint32_t H, uint32_t nViews);
 
+/* Creates a context on cfg->device.  On failure *out is null and nothing
+ * stays allocated. */
 int lhCreate(const LhConfig *cfg, LhContext **out);
 int lhDestroy(LhContext *ctx);
 

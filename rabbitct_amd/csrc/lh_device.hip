@@ -984,6 +984,8 @@ void lhDefaultConfig(LhConfig *cfg, uint32_t L, uint32_t W, uint32_t H, uint32_t
   cfg->device       = 0;
 }
 
+static int create_buffers(LhContext *c);
+
 int lhCreate(const LhConfig *cfg, LhContext **out)
 {
   *out = nullptr;
@@ -1006,7 +1008,19 @@ int lhCreate(const LhConfig *cfg, LhContext **out)
     return fail(LH_E_NOMEM, "host allocation failed");
   c->cfg           = *cfg;
   c->cfg.runLength = R;
-  *out             = c;
+  const int rc     = create_buffers(c);
+  if (rc != LH_OK) {
+    lhDestroy(c); // releases whatever was created; the error message stays
+    return rc;
+  }
+  *out = c;
+  return LH_OK;
+}
+
+// Device buffers, streams and the upload ring of a new context (c->cfg set).
+static int create_buffers(LhContext *c)
+{
+  const LhConfig *cfg = &c->cfg;
   LH_HIP(hipSetDevice(cfg->device));
   LH_HIP(hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, cfg->device));
   LH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1124,6 +1138,18 @@ int lhSetRangesFromGeometry(LhContext *c, const double *geometry, int vs)
     wacc[i] = acc;
   double *dGeom = nullptr;
   float *dW     = nullptr;
+  // freed on every exit, also when a step below fails
+  struct Scratch {
+    double *&g;
+    float *&w;
+    ~Scratch()
+    {
+      if (g)
+        (void)hipFree(g);
+      if (w)
+        (void)hipFree(w);
+    }
+  } scratch{dGeom, dW};
   LH_HIP(hipSetDevice(g.device));
   LH_HIP(hipMalloc(&dGeom, (size_t)g.nViews * 12 * sizeof(double)));
   LH_HIP(hipMalloc(&dW, g.L * sizeof(float)));
@@ -1138,8 +1164,6 @@ int lhSetRangesFromGeometry(LhContext *c, const double *geometry, int vs)
   LH_HIP(hipMemcpyAsync(host.data(), c->dRanges, host.size() * sizeof(int),
       hipMemcpyDeviceToHost, c->stream));
   LH_HIP(hipStreamSynchronize(c->stream));
-  (void)hipFree(dGeom);
-  (void)hipFree(dW);
   count_effective(c, host.data());
   return LH_OK;
 }
