@@ -79,13 +79,12 @@ constexpr int kWave          = 64;
 // Per-slot view parameters, wave-uniform in the kernel (scalar loads).
 struct alignas(16) ViewParams {
   double a[12];    // FP64 matrix, column-major a[c*3+r], times k = 2^-m
-  float af[12];    // the same in FP32
   float du, dv, dw; // a[0..2] * R_L in FP32: per-voxel step along x
   int id;          // 1-based view id (selects the line ranges)
   float scale;     // storage scale s = 4^m of the projection (1 for FP32)
   int pad[3];
 };
-static_assert(sizeof(ViewParams) == 176, "ViewParams layout");
+static_assert(sizeof(ViewParams) == 128, "ViewParams layout");
 
 struct BpArgs {
   float *vol;
@@ -207,9 +206,9 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
   const int z    = tz * kWaves + wv;
   const int L    = A.L;
 
-  const float fx0 = (float)((double)A.O_L + (double)x0 * (double)A.R_L);
-  const float fy  = (float)((double)A.O_L + (double)y * (double)A.R_L);
-  const float fz  = (float)((double)A.O_L + (double)z * (double)A.R_L);
+  const double wx0 = (double)A.O_L + (double)x0 * (double)A.R_L;
+  const double wy  = (double)A.O_L + (double)y * (double)A.R_L;
+  const double wz  = (double)A.O_L + (double)z * (double)A.R_L;
   const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
   const uint64_t LL   = (uint64_t)L * (uint64_t)L;
 
@@ -237,10 +236,10 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
       continue;
     touched |= active;
 
-    // the same FP32 run origin as the staged kernel
-    const float u0 = fmaf(P.af[0], fx0, fmaf(P.af[3], fy, fmaf(P.af[6], fz, P.af[9])));
-    const float v0 = fmaf(P.af[1], fx0, fmaf(P.af[4], fy, fmaf(P.af[7], fz, P.af[10])));
-    const float w0 = fmaf(P.af[2], fx0, fmaf(P.af[5], fy, fmaf(P.af[8], fz, P.af[11])));
+    // FP64 per-line origin, as in the staged kernel
+    const float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
+    const float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
+    const float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
     const float du = P.du, dv = P.dv, dw = P.dw;
     const __amdgpu_buffer_rsrc_t rs = slot_rsrc(A, slot);
 
@@ -518,10 +517,9 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   const int L    = A.L;
   const double O = A.O_L, RL = A.R_L;
 
-  // world coordinates of the run's first voxel, rounded once to FP32
-  const float fx0 = (float)(O + (double)x0 * RL);
-  const float fy  = (float)(O + (double)y * RL);
-  const float fz  = (float)(O + (double)z * RL);
+  const double wx0 = O + (double)x0 * RL;
+  const double wy  = O + (double)y * RL;
+  const double wz  = O + (double)z * RL;
   const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
   const uint64_t LL   = (uint64_t)L * (uint64_t)L;
   const int pitch     = (int)(A.rowBytes / ELEM);
@@ -595,13 +593,12 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     const bool active = rlo < rhi;
     if (blockAct && __any(active)) {
       touched |= active;
-      // run origin in FP32, as the reference's ISPC kernel evaluates its
-      // coordinates (fastRabbit.ispc:23-26; LolaOPT.c:139-147 rounds FP64
-      // partial sums to float): nine FMAs on scalar operands instead of FP64
-      // arithmetic and conversions
-      float u0 = fmaf(P.af[0], fx0, fmaf(P.af[3], fy, fmaf(P.af[6], fz, P.af[9])));
-      float v0 = fmaf(P.af[1], fx0, fmaf(P.af[4], fy, fmaf(P.af[7], fz, P.af[10])));
-      float w0 = fmaf(P.af[2], fx0, fmaf(P.af[5], fy, fmaf(P.af[8], fz, P.af[11])));
+      // FP64 per-line origin A (x0, y, z, 1), rounded once to FP32; the run's
+      // voxels step from it in FP32, one rounding each, so nothing drifts
+      // along x at any L
+      float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
+      float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
+      float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
       const float du = P.du, dv = P.dv, dw = P.dw;
       // Voxels outside their line's range are "parked": (u, v, w) = (pu, pv, 1)
       // * 2^100 puts the coordinate at (pu, pv), inside the footprint half a
@@ -787,10 +784,8 @@ __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     if (PREC != LH_PREC_F32) {
       const double k = ldexp(1.0, -m);
-      for (int c = 0; c < 12; ++c) {
+      for (int c = 0; c < 12; ++c)
         vp.a[c] *= k;
-        vp.af[c] *= (float)k;
-      }
       vp.du *= (float)k;
       vp.dv *= (float)k;
       vp.dw *= (float)k;
@@ -1261,10 +1256,8 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
     return fail(LH_E_ARG, "lhLoadView: slot %u / id %d out of range", slot, id);
   ViewParams vp;
   std::memset(&vp, 0, sizeof vp);
-  for (int k = 0; k < 12; ++k) {
-    vp.a[k]  = matrix[k];
-    vp.af[k] = (float)matrix[k];
-  }
+  for (int k = 0; k < 12; ++k)
+    vp.a[k] = matrix[k];
   vp.du    = (float)(matrix[0] * (double)g.R_L);
   vp.dv    = (float)(matrix[1] * (double)g.R_L);
   vp.dw    = (float)(matrix[2] * (double)g.R_L);
