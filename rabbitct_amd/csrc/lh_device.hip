@@ -288,8 +288,8 @@ __device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
 // than kFW x kFH falls back to L2 gathers for that view.
 // updates per scheduling group in the staged loop, by precision (measured
 // at L=512 with R=16: FP32 fastest with groups of 16, that is one fence at
-// the end of the run, 45.7 against 47.2 ms per step with groups of 4; the
-// half-precision kernels fastest with 4)
+// the end of the run, 47.4 against 48.5 ms per step with groups of 2 or 4;
+// the half-precision kernels fastest with 4)
 template <int PREC>
 constexpr int kFence = PREC == LH_PREC_F32 ? 16 : 4;
 // updates per scheduling group of the L2-gather fallback, by run length
