@@ -59,6 +59,8 @@ def make_context(args, geom, phantom, ranges, kernel, prec, vpp):
     ctx = lolahip.Context(args.L, geom.width, geom.height, geom.n_views, n_slots=geom.n_views,
                           precision=prec, kernel=kernel, views_per_pass=vpp,
                           run_length=args.run)
+    if args.slab:
+        ctx.set_slab(*args.slab)
     ctx.set_ranges(ranges)
     generate_into(ctx, geom, phantom)
     ctx.synchronize()
@@ -68,6 +70,16 @@ def make_context(args, geom, phantom, ranges, kernel, prec, vpp):
         tuned = autotune.autotune(ctx, min(args.tune_views, geom.n_views), prec)
         ctx.zero_volume()
     return ctx, tuned
+
+
+def slab_effective(ranges, args):
+    """Effective updates (voxels inside their line's range) of the z-slab over
+    all views, from the host copy of the ranges ([view][z*L+y] start, stop)."""
+    z0, z1 = args.slab
+    r = np.asarray(ranges).reshape(args.views, args.L, args.L, 2)[:, z0:z1].astype(np.int64)
+    start = np.maximum(r[..., 0], 0)
+    stop = np.minimum(r[..., 1], args.L)
+    return int(np.maximum(stop - start, 0).sum())
 
 
 def time_variant(args, ctx, n_views):
@@ -222,7 +234,13 @@ def main():
                     help="phantom ellipsoids (30 for the L=1024 configuration)")
     ap.add_argument("--no-sparse-cnn", action="store_true",
                     help="skip the sparse-view + CNN volumes/s measurement at L=256")
+    ap.add_argument("--slab", default="",
+                    help="Z0:Z1: backproject only that z-slab (one GPU's share of a volume "
+                         "sharded into z-slabs; multiples of 8)")
     args = ap.parse_args()
+    if args.slab:
+        z0, z1 = (int(v) for v in args.slab.split(":"))
+        args.slab = (z0, z1)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the z-slab sharding is a design only); run with --gpus 1", file=sys.stderr)
@@ -247,13 +265,14 @@ def main():
         log(f"{var}: context + {args.views} views generated (+ tuning) in "
             f"{time.time() - t0:.1f} s; shape {tuned['best'] if tuned else 'plain'}")
         ms, ms_launch, launches, vol = time_variant(args, ctx, args.views)
-        eff = ctx.effective_updates(args.views)
+        eff = slab_effective(ranges, args) if args.slab else ctx.effective_updates(args.views)
         if args.calib_copy_mib:
             cms = ctx.copy_calibration(args.calib_copy_mib << 20)
             log(f"calibration copy {args.calib_copy_mib} MiB: {cms:.3f} ms "
                 f"({2 * (args.calib_copy_mib << 20) / cms / 1e6:.0f} GB/s read+write)")
         ctx.close()
-        nominal = args.L ** 3 * args.views
+        depth = args.slab[1] - args.slab[0] if args.slab else args.L
+        nominal = depth * args.L ** 2 * args.views
         results[var] = {"ms_per_step": round(ms, 3), "gups": round(nominal / ms / 1e6, 2),
                         "gups_effective": round(eff / ms / 1e6, 2),
                         "ms_per_launch": round(ms_launch, 4), "launches": launches,
@@ -282,7 +301,7 @@ def main():
     line = {
         "metric": METRIC,
         "value": head["gups"],
-        "unit": "GUPS (L^3 x views / s)",
+        "unit": "GUPS (slab voxels x views / s)" if args.slab else "GUPS (L^3 x views / s)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -293,8 +312,9 @@ def main():
         "dtype": "f32",
         "data": f"synthetic: seeded {args.ellipsoids}-ellipsoid phantom (seed {SEED}), FDK-filtered analytic "
                 "projections generated on the GPU",
-        "config": {"workload": f"backprojection L={args.L} (R_L={voxel_size(args.L)} mm), "
-                               f"{args.views} views of "
+        "config": {"workload": f"backprojection L={args.L} (R_L={voxel_size(args.L)} mm"
+                               + (f", z-slab {args.slab[0]}:{args.slab[1]}" if args.slab else "")
+                               + f"), {args.views} views of "
                                f"{geom.width}x{geom.height} float32, line clipping VECTORSIZE 16, "
                                f"{args.headline} b={vpp}",
                    "L": args.L, "views": args.views, "detector": [geom.width, geom.height],

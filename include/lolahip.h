@@ -208,6 +208,12 @@ int lhSetGraphs(LhContext *ctx, int enable);
 int lhSetLaunch(LhContext *ctx, int kernel, uint32_t viewsPerPass, uint32_t runLength,
     int staging);
 
+/* Restricts later backprojections to the z-slab [z0, z1) of the volume
+ * (multiples of 8; [0, L) after lhCreate): one GPU's share of a volume
+ * sharded into z-slabs.  Voxels outside the slab are left untouched.  No
+ * counterpart in the reference, whose loops always cover the whole volume. */
+int lhSetSlab(LhContext *ctx, uint32_t z0, uint32_t z1);
+
 /* Device-to-device copy of `bytes` on the context's stream, timed with HIP
  * events: the reference copy against which HBM counters are scaled. */
 int lhCopyCalibration(LhContext *ctx, uint64_t bytes, float *ms);

@@ -101,6 +101,7 @@ struct BpArgs {
   int firstSlot, nViews;
   int clip;
   int tilesX, tilesY, tilesZ; // tile grid (x-runs, 64-line groups, 4-slice groups)
+  int tileZ0;              // first z tile of the launch (a z-slab: tiles [tileZ0, +tilesZ))
   int persistent;
   const int2 *boxes;       // staged kernel: footprint box per (tile, slot), [tile][boxSlots]
   int boxSlots;            // slots per tile row of boxes
@@ -700,9 +701,9 @@ __global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpA
   __shared__ int sAct[2 * kWaves];
   if (!A.persistent) {
     if constexpr (STAGE)
-      bp_tile_lds<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z, sImg, sAct);
+      bp_tile_lds<R, PREC>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z, sImg, sAct);
     else
-      bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+      bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z);
     return;
   }
   const int nTiles  = A.tilesX * A.tilesY * A.tilesZ;
@@ -716,7 +717,7 @@ __global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpA
       break;
     const int tx = t % A.tilesX;
     const int ty = (t / A.tilesX) % A.tilesY;
-    const int tz = t / (A.tilesX * A.tilesY);
+    const int tz = A.tileZ0 + t / (A.tilesX * A.tilesY);
     if constexpr (STAGE) {
       __syncthreads(); // the previous tile's last LDS reads are done
       bp_tile_lds<R, PREC>(A, tx, ty, tz, sImg, sAct);
@@ -915,6 +916,7 @@ struct LhContext {
   int numCU = 0;
   std::vector<uint64_t> effPerView; // by id-1
   std::vector<int> slotId;          // id held by each slot (0 = empty)
+  uint32_t slabZ0 = 0, slabZ1 = 0;  // z-slab of the volume backprojected ([0, L) by default)
   int2 *dBoxes  = nullptr;          // staged kernel: footprint box per (slot, tile)
   long boxTiles = 0;                // tiles per slot in dBoxes
   std::vector<int> boxR;            // run length the slot's boxes were built for (0 = none)
@@ -1053,6 +1055,8 @@ static int create_buffers(LhContext *c)
     LH_HIP(hipEventCreateWithFlags(&c->copyDone[k], hipEventDisableTiming));
     LH_HIP(hipEventCreateWithFlags(&c->stageDone[k], hipEventDisableTiming));
   }
+  c->slabZ0 = 0;
+  c->slabZ1 = cfg->L;
   c->effPerView.assign(cfg->nViews, cfg->clip ? 0 : L * L * L);
   c->slotId.assign(cfg->nSlots, 0);
   c->boxR.assign(cfg->nSlots, 0);
@@ -1446,12 +1450,17 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   A.tilesX      = (int)g.L / R;
   A.tilesY      = (int)g.L / (stage ? kLYs : kWave);
   A.tilesZ      = (int)g.L / (stage ? kTileZ : kWaves);
+  // the box table covers every z tile; a slab launch walks its own
+  const int tilesZAll = A.tilesZ;
+  const int zTile     = stage ? kTileZ : kWaves;
+  A.tileZ0      = (int)c->slabZ0 / zTile;
+  A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / zTile;
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
   A.boxes       = nullptr;
   A.boxSlots    = (int)g.nSlots;
   A.boxBytes    = 0;
   if (stage) {
-    const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, A.tilesZ, forceBoxes);
+    const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, tilesZAll, forceBoxes);
     if (rc != LH_OK)
       return rc;
     A.boxes    = c->dBoxes;
@@ -1612,6 +1621,18 @@ int lhSetLaunch(LhContext *c, int kernel, uint32_t viewsPerPass, uint32_t runLen
   c->cfg.viewsPerPass = viewsPerPass;
   c->cfg.runLength    = runLength;
   c->cfg.staging      = staging ? 1 : 0;
+  return LH_OK;
+}
+
+int lhSetSlab(LhContext *c, uint32_t z0, uint32_t z1)
+{
+  if (!c || z0 >= z1 || z1 > c->cfg.L || z0 % 8 || z1 % 8)
+    return fail(LH_E_ARG, "lhSetSlab: need 0 <= z0 < z1 <= L, multiples of 8 (got %u, %u)", z0,
+        z1);
+  if (z0 != c->slabZ0 || z1 != c->slabZ1)
+    drop_graphs(c); // captured launches hold the old tile range
+  c->slabZ0 = z0;
+  c->slabZ1 = z1;
   return LH_OK;
 }
 

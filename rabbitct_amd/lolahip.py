@@ -74,7 +74,7 @@ EXPORTS = (
     "lhLoadView", "lhLoadViewDevice", "lhBackproject", "lhTimedBackproject",
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
     "lhSetLaunch", "lhCopyCalibration", "lhVolumeDevice", "lhScaleVolume",
-    "lhAcquireUpload", "lhCommitUpload", "lhCancelUpload", "lhSetGraphs",
+    "lhAcquireUpload", "lhCommitUpload", "lhCancelUpload", "lhSetGraphs", "lhSetSlab",
     # sparse-view + learned correction
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhCommitUpload.argtypes = [P, u32, i32, P]
     lib.lhCancelUpload.argtypes = [P]
     lib.lhSetGraphs.argtypes = [P, i32]
+    lib.lhSetSlab.argtypes = [P, u32, u32]
     u64, f64 = ctypes.c_uint64, ctypes.c_double
     lib.lhLineIntegrals.argtypes = [i32, i32, i32, u32, P, P, u32, P, P]
     lib.lhForwardProject.argtypes = [P, i32, f32, f32, i32, i32, i32, u32, P, P, f32, P]
@@ -279,6 +280,11 @@ class Context:
     def set_graphs(self, enable: bool) -> None:
         _check(self.lib.lhSetGraphs(self._h, 1 if enable else 0), "lhSetGraphs")
 
+    def set_slab(self, z0: int, z1: int) -> None:
+        """Backproject only the z-slab [z0, z1) of the volume (one GPU's share
+        of a z-sharded volume); multiples of 8."""
+        _check(self.lib.lhSetSlab(self._h, z0, z1), "lhSetSlab")
+
     def scale_volume(self, s: float) -> None:
         _check(self.lib.lhScaleVolume(self._h, s), "lhScaleVolume")
 
@@ -346,14 +352,17 @@ class Context:
 
 
 def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel="fused",
-                      views_per_pass=32, run_length=16, volume=None, staging=True):
+                      views_per_pass=32, run_length=16, volume=None, staging=True, slab=None):
     """Backproject views (N,12)/(N,H,W) on the GPU; returns the (L,L,L) float32
-    volume (+= into ``volume`` when given).  ``ranges`` None = unclipped."""
+    volume (+= into ``volume`` when given).  ``ranges`` None = unclipped;
+    ``slab`` (z0, z1) restricts the work to that z-slab."""
     images = np.asarray(images, dtype=np.float32)
     N, H, W = images.shape
     with Context(L, W, H, N, n_slots=N, precision=precision, kernel=kernel,
                  views_per_pass=views_per_pass, run_length=run_length,
                  clip=ranges is not None, staging=staging) as ctx:
+        if slab is not None:
+            ctx.set_slab(*slab)
         if ranges is not None:
             ctx.set_ranges(ranges)
         if volume is not None:

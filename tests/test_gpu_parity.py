@@ -218,6 +218,31 @@ def test_staged_equals_l2_gathers_bitwise(prec, run, noise):
     assert diff == 0
 
 
+@pytest.mark.parametrize("kernel,staging", [("fused", True), ("fused", False), ("plain", True)])
+def test_z_slab_equals_full_volume_slice(kernel, staging):
+    """One GPU's share of a z-sharded volume: a launch restricted to the slab
+    [z0, z1) gives, inside the slab, the voxels of the full backprojection
+    bit for bit, and leaves every voxel outside it untouched (zero).  L=256,
+    a central slab, an off-centre one and the first slices, 3 full-detector
+    views."""
+    geom, imgs, ranges, _ = _full_detector_case(256, 3, 37, noise=False)
+    kw = dict(kernel=kernel, views_per_pass=3, run_length=16, staging=staging)
+    full = lolahip.backproject_views(256, geom.matrices, imgs, ranges, **kw)
+    assert full[96:160].any() and full[40:72].any()
+    for z0, z1 in ((96, 160), (40, 72), (0, 8)):
+        part = lolahip.backproject_views(256, geom.matrices, imgs, ranges, slab=(z0, z1), **kw)
+        assert np.array_equal(part[z0:z1].view(np.uint32), full[z0:z1].view(np.uint32))
+        assert not part[:z0].any() and not part[z1:].any()
+
+
+def test_z_slab_rejects_bad_bounds():
+    with lolahip.Context(128, 32, 24, 1, n_slots=1) as ctx:
+        for z0, z1 in ((8, 8), (4, 16), (0, 136), (16, 8)):
+            with pytest.raises(lolahip.LolaHipError):
+                ctx.set_slab(z0, z1)
+        ctx.set_slab(0, 128)
+
+
 def test_unclipped_l512_staged_bitwise_and_vs_fp64():
     """Unclipped mode (every voxel of every line, LolaBunny's loop) at L=512
     on 3 full-detector views: the LDS-staged and L2-gather fused kernels give
