@@ -82,6 +82,18 @@ def slab_effective(ranges, args):
     return int(np.maximum(stop - start, 0).sum())
 
 
+def gather_roof(head):
+    """The backprojector's inner step without the projection (lhGatherRoof),
+    taps from LDS and from L2, 4 workgroups per CU as the staged kernel runs:
+    the practical roof of the gather, against which the headline's effective
+    rate is set."""
+    lds = lolahip.gather_roof(True) / 1e9
+    l2 = lolahip.gather_roof(False) / 1e9
+    log(f"gather roof: LDS taps {lds:.0f}, L2 taps {l2:.0f} G updates/s")
+    return {"unit": "G updates/s", "lds": round(lds, 1), "l2": round(l2, 1),
+            "headline_effective_over_lds": round(head["gups_effective"] / lds, 3)}
+
+
 def time_variant(args, ctx, n_views):
     """Warm-up reps, one rep whose volume is kept for the error report, then
     the timed reps (each rep = one step = all views)."""
@@ -326,6 +338,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "bytes_per_update": bytes_per_update,
                      "updates_per_launch": int(eff / launches)},
+        "gather_roof": gather_roof(head),
         "cpu_baseline": None,
         "sparse_cnn": None if args.no_sparse_cnn else sparse_cnn(args),
     }

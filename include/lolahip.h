@@ -214,6 +214,13 @@ int lhSetLaunch(LhContext *ctx, int kernel, uint32_t viewsPerPass, uint32_t runL
  * counterpart in the reference, whose loops always cover the whole volume. */
 int lhSetSlab(LhContext *ctx, uint32_t z0, uint32_t z1);
 
+/* Gather roof: updates/s of the backprojector's inner step without the
+ * projection (coordinate step, floor/fraction, four taps, bilinear lerp,
+ * accumulate) over 2D-local lane paths, taps from LDS (lds = 1) or from an
+ * L2-resident image; blocksPerCU workgroups of 256 lanes per CU, `passes`
+ * runs of 16 updates per lane (<= 1000).  No counterpart in the reference. */
+int lhGatherRoof(int lds, int blocksPerCU, int passes, double *updatesPerSec, float *ms);
+
 /* Device-to-device copy of `bytes` on the context's stream, timed with HIP
  * events: the reference copy against which HBM counters are scaled. */
 int lhCopyCalibration(LhContext *ctx, uint64_t bytes, float *ms);

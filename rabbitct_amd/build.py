@@ -5,6 +5,7 @@
   lh_unet.hip    -> hipcc (3D U-Net: MFMA convolutions, backward, Adam)
   lh_learned.hip -> hipcc (patch gather, forward projector, residual)
   lh_metric.hip  -> hipcc (the -c HU metric on the GPU)
+  lh_roof.hip    -> hipcc (gather roof microbenchmark)
   lh_module.c    -> gcc -fopenmp (RabbitCT module, clip files, CPU ranges)
   link           -> hipcc -shared, in-tree: rabbitct_amd/liblolahip.so
 """
@@ -27,7 +28,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-I" + INCLU
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wall"]
 HIP_SOURCES = ["lh_device.hip", "lh_phantom.hip", "lh_unet.hip", "lh_learned.hip",
-               "lh_metric.hip"]
+               "lh_metric.hip", "lh_roof.hip"]
 C_SOURCES = ["lh_module.c"]
 C_FLAGS = ["-O3", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wextra", "-I" + INCLUDE]
 

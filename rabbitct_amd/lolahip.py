@@ -75,6 +75,7 @@ EXPORTS = (
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
     "lhSetLaunch", "lhCopyCalibration", "lhVolumeDevice", "lhScaleVolume",
     "lhAcquireUpload", "lhCommitUpload", "lhCancelUpload", "lhSetGraphs", "lhSetSlab",
+    "lhGatherRoof",
     # sparse-view + learned correction
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
@@ -141,6 +142,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhCancelUpload.argtypes = [P]
     lib.lhSetGraphs.argtypes = [P, i32]
     lib.lhSetSlab.argtypes = [P, u32, u32]
+    lib.lhGatherRoof.argtypes = [i32, i32, i32, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(f32)]
     u64, f64 = ctypes.c_uint64, ctypes.c_double
     lib.lhLineIntegrals.argtypes = [i32, i32, i32, u32, P, P, u32, P, P]
     lib.lhForwardProject.argtypes = [P, i32, f32, f32, i32, i32, i32, u32, P, P, f32, P]
@@ -349,6 +352,15 @@ class Context:
 
     def effective_updates(self, count):
         return int(self.lib.lhEffectiveUpdates(self._h, count))
+
+
+def gather_roof(lds: bool, blocks_per_cu: int = 4, passes: int = 64) -> float:
+    """Updates/s of the gather roof microbenchmark (lhGatherRoof)."""
+    lib = load_library()
+    r, ms = ctypes.c_double(), ctypes.c_float()
+    _check(lib.lhGatherRoof(1 if lds else 0, blocks_per_cu, passes, ctypes.byref(r),
+                            ctypes.byref(ms)), "lhGatherRoof")
+    return r.value
 
 
 def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel="fused",

@@ -454,3 +454,12 @@ def test_reference_runner_verifies_lolahip(tmp_path, golden_border):
           f"{f('Max. Absolute Error')}")
     assert f("Mean Squared Error") == pytest.approx(mse, rel=1e-5, abs=1e-12)
     assert f("Max. Absolute Error") == float(np.abs(err).max()) <= 1.0
+
+
+def test_gather_roof_runs_and_checks_arguments():
+    """The gather-roof microbenchmark reports a positive rate for both tap
+    sources and rejects arguments that would leave its LDS window."""
+    for lds in (True, False):
+        assert lolahip.gather_roof(lds, 4, 8) > 0.0
+    with pytest.raises(lolahip.LolaHipError):
+        lolahip.gather_roof(True, 4, 1001)
