@@ -124,33 +124,41 @@ def cpu_baseline(args, geom, phantom, ranges):
     except AttributeError:
         pass
     out = {}
+    n = n_asm
     with tempfile.TemporaryDirectory() as d:
-        for alg, n in (("LolaASM", n_asm), ("LolaOPT", n_opt)):
-            hdr = rct.RctHeader(geom.width, geom.height, n, (1000.0, 0.0))
-            rct.write_rct(os.path.join(d, "in.rct"), hdr, geom.matrices[:n], imgs[:n])
-            rct.write_geometry(os.path.join(d, "geo.rct"), hdr, geom.matrices[:n])
-            rct.write_clip_file(os.path.join(d, "clip.rct"), ranges[:n])
-            env = {**os.environ, "OMP_NUM_THREADS": str(cores), "OMP_PROC_BIND": "close",
-                   "OMP_PLACES": "cores", "OMP_STACKSIZE": "64M"}
-            log(f"cpu baseline: {alg} ({isa}) on {n} views, {cores} threads")
+        hdr = rct.RctHeader(geom.width, geom.height, n, (1000.0, 0.0))
+        rct.write_rct(os.path.join(d, "in.rct"), hdr, geom.matrices[:n], imgs[:n])
+        rct.write_geometry(os.path.join(d, "geo.rct"), hdr, geom.matrices[:n])
+        rct.write_clip_file(os.path.join(d, "clip.rct"), ranges[:n])
+        env = {**os.environ, "OMP_NUM_THREADS": str(cores), "OMP_PROC_BIND": "close",
+               "OMP_PLACES": "cores", "OMP_STACKSIZE": "64M"}
+        # -b 1 as the reference's own benchmark script runs it, and LolaASM
+        # with -b 16, which its y x view blocking uses
+        for alg, b in (("LolaASM", 1), ("LolaOPT", 1), ("LolaASM", 16)):
+            if n % b:
+                continue
+            log(f"cpu baseline: {alg} ({isa}) on {n} views, -b {b}, {cores} threads")
             p = subprocess.run([runner, "-i", "in.rct", "-a", "geo.rct", "-m", alg, "-s",
-                                str(args.L), "-b", "1", "-C", "clip.rct"], cwd=d, env=env,
+                                str(args.L), "-b", str(b), "-C", "clip.rct"], cwd=d, env=env,
                                capture_output=True, text=True, timeout=900)
             if p.returncode != 0:
                 log(p.stdout[-2000:] + p.stderr[-2000:])
                 return None
             total = float(re.search(r"Total: ([0-9.eE+-]+) s", p.stdout).group(1))
-            out[alg] = {"seconds": total, "views": n,
-                        "gups": args.L ** 3 * n / total / 1e9}
-    asm = out["LolaASM"]
+            out[(alg, b)] = {"seconds": total, "views": n,
+                             "gups": args.L ** 3 * n / total / 1e9}
+    asm = out[("LolaASM", 1)]
+    opt = out[("LolaOPT", 1)]
+    asm16 = out.get(("LolaASM", 16))
     return {"value": round(asm["gups"], 3), "unit": "GUPS", "cores": cores,
             "kind": "reference",
             "sample": f"LolaASM ({isa}, VECTORSIZE {16 if isa == 'avx512' else 8}) over the "
                       f"first {asm['views']} of the {geom.n_views} views at L={args.L}, -b 1, "
                       f"reference runner Total {asm['seconds']:.3f} s",
-            "lolaopt_gups": round(out["LolaOPT"]["gups"], 3),
-            "lolaopt_sample": f"first {out['LolaOPT']['views']} views, Total "
-                              f"{out['LolaOPT']['seconds']:.3f} s"}
+            "lolaopt_gups": round(opt["gups"], 3),
+            "lolaopt_sample": f"first {opt['views']} views, -b 1, Total "
+                              f"{opt['seconds']:.3f} s",
+            "lolaasm_b16_gups": round(asm16["gups"], 3) if asm16 else None}
 
 
 def sparse_cnn(args):
