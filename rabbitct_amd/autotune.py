@@ -13,7 +13,7 @@ import os
 from . import lolahip
 
 RUNS = (8, 16, 32)
-PASSES = (16, 32, 64)
+PASSES = (8, 16, 32, 64)
 
 
 def candidates(precision: str):
