@@ -210,8 +210,9 @@ int lhSetLaunch(LhContext *ctx, int kernel, uint32_t viewsPerPass, uint32_t runL
 
 /* Restricts later backprojections to the z-slab [z0, z1) of the volume
  * (multiples of 8; [0, L) after lhCreate): one GPU's share of a volume
- * sharded into z-slabs.  Voxels outside the slab are left untouched.  No
- * counterpart in the reference, whose loops always cover the whole volume. */
+ * sharded into z-slabs.  Voxels outside the slab are left untouched, and
+ * lhEffectiveUpdates still counts the whole volume.  No counterpart in the
+ * reference, whose loops always cover the whole volume. */
 int lhSetSlab(LhContext *ctx, uint32_t z0, uint32_t z1);
 
 /* Gather roof: updates/s of the backprojector's inner step without the
