@@ -218,21 +218,49 @@ def test_staged_equals_l2_gathers_bitwise(prec, run, noise):
     assert diff == 0
 
 
-@pytest.mark.parametrize("kernel,staging", [("fused", True), ("fused", False), ("plain", True)])
-def test_z_slab_equals_full_volume_slice(kernel, staging):
+@pytest.mark.parametrize("kernel,staging,prec", [("fused", True, "f32"), ("fused", False, "f32"),
+                                                ("plain", True, "f32"), ("fused", True, "bf16")])
+def test_z_slab_equals_full_volume_slice(kernel, staging, prec):
     """One GPU's share of a z-sharded volume: a launch restricted to the slab
     [z0, z1) gives, inside the slab, the voxels of the full backprojection
     bit for bit, and leaves every voxel outside it untouched (zero).  L=256,
     a central slab, an off-centre one and the first slices, 3 full-detector
     views."""
     geom, imgs, ranges, _ = _full_detector_case(256, 3, 37, noise=False)
-    kw = dict(kernel=kernel, views_per_pass=3, run_length=16, staging=staging)
+    kw = dict(kernel=kernel, views_per_pass=3, run_length=16, staging=staging, precision=prec)
     full = lolahip.backproject_views(256, geom.matrices, imgs, ranges, **kw)
     assert full[96:160].any() and full[40:72].any()
     for z0, z1 in ((96, 160), (40, 72), (0, 8)):
         part = lolahip.backproject_views(256, geom.matrices, imgs, ranges, slab=(z0, z1), **kw)
         assert np.array_equal(part[z0:z1].view(np.uint32), full[z0:z1].view(np.uint32))
         assert not part[:z0].any() and not part[z1:].any()
+
+
+def test_slab_effective_updates_match_the_library():
+    """bench.py counts a slab's effective updates from the host ranges; over
+    the whole volume that count equals the library's own (lhEffectiveUpdates)."""
+    import types
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "bench", os.path.join(os.path.dirname(os.path.dirname(__file__)), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    L, N = 128, 4
+    geom = ScanGeometry(n_views=N, seed=41).build()
+    ranges = lolahip.compute_line_ranges(L, geom.width, geom.height, geom.matrices, 16)
+    blank = np.zeros((geom.height, geom.width), np.float32)
+    with lolahip.Context(L, geom.width, geom.height, N, n_slots=N) as ctx:
+        ctx.set_ranges(ranges)
+        for v in range(N):
+            ctx.load_view(v, v + 1, geom.matrices[v], blank)
+        lib_count = ctx.effective_updates(N)
+    assert lib_count > 0
+    args = types.SimpleNamespace(slab=(0, L), views=N, L=L)
+    assert bench.slab_effective(ranges, args) == lib_count
+    args.slab = (0, 64)
+    part = bench.slab_effective(ranges, args)
+    args.slab = (64, L)
+    assert part + bench.slab_effective(ranges, args) == lib_count
 
 
 def test_z_slab_rejects_bad_bounds():
