@@ -491,3 +491,51 @@ def test_gather_roof_runs_and_checks_arguments():
         assert lolahip.gather_roof(lds, 4, 8) > 0.0
     with pytest.raises(lolahip.LolaHipError):
         lolahip.gather_roof(True, 4, 1001)
+
+
+@pytest.fixture(scope="module")
+def l512_views():
+    """32 views of the L=512 scan geometry (every 16th of 496, full
+    1248 x 960 detector, seeded phantom) with their clip ranges."""
+    from rabbitct_amd.generate import generate_into
+    geom = ScanGeometry(n_views=496, seed=512).build()
+    sel = np.arange(0, 496, 16)
+    with lolahip.Context(512, geom.width, geom.height, 496, n_slots=len(sel)) as ctx:
+        imgs = generate_into(ctx, geom, EllipsoidPhantom(seed=512).build(), views=sel,
+                             want_host=True, ids=sel + 1)
+    mats = geom.matrices[sel]
+    ranges = lolahip.compute_line_ranges(512, geom.width, geom.height, mats, 16)
+    return mats, imgs, ranges
+
+
+@pytest.mark.parametrize("kernel,prec", [("fused", "f32"), ("plain", "f32"), ("fused", "bf16")])
+def test_l512_exact_scaling_invariants(l512_views, kernel, prec):
+    """Size-independent properties at the full L=512 size, exact in floating
+    point because every factor is a power of two: images x 2 give the
+    volume x 2, and the matrices x 2 (u, v, w all doubled: the same detector
+    coordinates, weight 1/w^2 quartered) give the volume / 4, bit for bit."""
+    mats, imgs, ranges = l512_views
+    kw = dict(kernel=kernel, precision=prec, views_per_pass=16, run_length=16)
+    base = lolahip.backproject_views(512, mats, imgs, ranges, **kw)
+    assert np.abs(base).max() > 0
+    twice = lolahip.backproject_views(512, mats, 2.0 * imgs, ranges, **kw)
+    assert np.array_equal(twice.view(np.uint32), (2.0 * base).view(np.uint32))
+    quarter = lolahip.backproject_views(512, 2.0 * mats, imgs, ranges, **kw)
+    assert np.array_equal(quarter.view(np.uint32), (base / 4.0).view(np.uint32))
+
+
+def test_l512_view_split_and_order(l512_views):
+    """Backprojection is a sum over views: two launches over the halves of the
+    views (accumulating into the volume) and one launch over the views in
+    reverse order equal the single launch to FP32 rounding (1e-6 of the
+    range), at the full L=512 size."""
+    mats, imgs, ranges = l512_views
+    kw = dict(kernel="fused", views_per_pass=32, run_length=16)
+    whole = lolahip.backproject_views(512, mats, imgs, ranges, **kw)
+    h = len(mats) // 2
+    first = lolahip.backproject_views(512, mats[:h], imgs[:h], ranges[:h], **kw)
+    split = lolahip.backproject_views(512, mats[h:], imgs[h:], ranges[h:], volume=first, **kw)
+    rev = lolahip.backproject_views(512, mats[::-1], imgs[::-1], ranges[::-1], **kw)
+    span = float(whole.max() - whole.min())
+    for other in (split, rev):
+        assert float(np.abs(other - whole).max()) <= 1e-6 * span
