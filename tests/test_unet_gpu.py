@@ -6,6 +6,8 @@ bfloat16, FP32 accumulation, one bfloat16 rounding of the output) within
 2^-8 relative of FP64; weight gradients (FP32 accumulation) within 1e-4 of
 their max; the whole net and its gradients, against the oracle run with the
 same bfloat16 storage points, within 1e-2 / 3e-2 relative L2."""
+import os
+
 import numpy as np
 import pytest
 
@@ -236,3 +238,24 @@ def test_relative_residual():
         r = unet.relative_residual(da, db, a.size)
     ref = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
     assert abs(r - ref) <= 1e-12 * ref
+
+
+def test_forward_is_translation_equivariant_d128():
+    """A property that needs no oracle, at a larger size than the oracle runs:
+    shifting the input by (8, 4, 16) voxels in (z, y, x) -- whole convolution
+    tiles and pooling cells -- shifts the output by the same amount, bit for
+    bit, wherever neither copy sees the zero border (32 voxels margin; the
+    receptive field is about 24).  D=128, trained weights."""
+    rng = np.random.default_rng(5)
+    D = 128
+    x = rng.normal(size=(1, D, D, D)).astype(np.float32)
+    xs = np.zeros_like(x)
+    xs[:, 8:, 4:, 16:] = x[:, :-8, :-4, :-16]
+    with unet.UNet(seed=0) as net:
+        net.load(os.path.join(unet.WEIGHTS_DIR, "unet_sparse4_l256.npz"))
+        y = net.forward_host(x)
+        ys = net.forward_host(xs)
+    a = y[:, 32:80, 32:88, 32:96]
+    b = ys[:, 40:88, 36:92, 48:112]
+    assert np.abs(a).max() > 0
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
