@@ -694,8 +694,13 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 // its L2 serves one band of detector rows.
 // A must stay the kernel's only argument: kernel_args() reads it back from
 // offset 0 of the kernarg segment.
-template <int R, int PREC, int STAGE>
-__global__ __launch_bounds__(kWaves *kWave, (R > 16 ? 2 : 4)) void bp_kernel(BpArgs A)
+// OCC: workgroups per CU the register budget is sized for (4: 128 VGPRs).
+// 32-voxel runs hold 32 accumulators; with OCC = 2 they may use more
+// registers, which their L2-gather fallback needs to keep four voxels' taps in
+// flight; the launcher picks OCC = 4 for them where footprints fit the LDS
+// window (small tiles).
+template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : 4)>
+__global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
 {
   __shared__ float sImg[STAGE ? 2 * kPitch * kFH : 1];
   __shared__ int sAct[2 * kWaves];
@@ -924,20 +929,20 @@ struct LhContext {
 
 // Grid of one launch: one workgroup per tile, or for the persistent form as
 // many workgroups as the chip keeps resident (occupancy x CUs).
-template <int R, int PREC, int STAGE>
+template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : 4)>
 static void launch_shape(LhContext *c, const BpArgs &A)
 {
   static int perCU = 0;
   dim3 grid(A.tilesX, A.tilesY, A.tilesZ);
   if (A.persistent) {
     if (perCU == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU,
-                           bp_kernel<R, PREC, STAGE>, kWaves * kWave, 0) != hipSuccess ||
+                           bp_kernel<R, PREC, STAGE, OCC>, kWaves * kWave, 0) != hipSuccess ||
                           perCU < 1))
       perCU = 4;
     grid = dim3((unsigned)std::min<long>((long)c->numCU * perCU,
         (long)A.tilesX * A.tilesY * A.tilesZ));
   }
-  bp_kernel<R, PREC, STAGE><<<grid, dim3(kWaves * kWave), 0, c->stream>>>(A);
+  bp_kernel<R, PREC, STAGE, OCC><<<grid, dim3(kWaves * kWave), 0, c->stream>>>(A);
 }
 
 const LhConfig &lh_config(const LhContext *c) { return c->cfg; }
@@ -1479,9 +1484,25 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
     if (stage) LH_LAUNCH(RR, LH_PREC_F16, 1); else LH_LAUNCH(RR, LH_PREC_F16, 0);          \
     break;                                                                               \
   }
+  // A staged 32-voxel run whose tile spans at most 8 mm along x (L=1024 and
+  // beyond) has footprints that fit the LDS window like the 16-voxel tiles
+  // at L=512: it runs at 4 workgroups per CU (128 VGPRs), 11% faster at
+  // L=1024.  Larger tiles (L <= 512) take the L2 fallback often, which is
+  // 1.5x slower within 128 VGPRs, so they keep the larger register budget.
+  const bool compact32 = stage && 32.0f * g.R_L <= 8.0f;
   switch (R) {
   case 8: LH_BY_PREC(8); break;
-  case 32: LH_BY_PREC(32); break;
+  case 32:
+    if (compact32) {
+      switch (g.precision) {
+      case LH_PREC_F32: launch_shape<32, LH_PREC_F32, 1, 4>(c, A); break;
+      case LH_PREC_BF16: launch_shape<32, LH_PREC_BF16, 1, 4>(c, A); break;
+      default: launch_shape<32, LH_PREC_F16, 1, 4>(c, A); break;
+      }
+    } else {
+      LH_BY_PREC(32);
+    }
+    break;
   default: LH_BY_PREC(16); break;
   }
 #undef LH_BY_PREC

@@ -218,6 +218,25 @@ def test_staged_equals_l2_gathers_bitwise(prec, run, noise):
     assert diff == 0
 
 
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_l1024_compact_run32_staged_equals_l2_gathers_bitwise(prec):
+    """At L=1024 a 32-voxel run's tile spans 8 mm along x, so the staged
+    kernel runs in its 128-VGPR form (4 workgroups per CU); it must still
+    equal the same computation with taps from L2 bit for bit.  3 views of
+    the full detector, noise images."""
+    geom = ScanGeometry(n_views=496, seed=43).build().subset(np.array([0, 165, 330]))
+    rng = np.random.default_rng(43)
+    imgs = rng.normal(size=(3, geom.height, geom.width)).astype(np.float32)
+    imgs[:, 0, :] = imgs[:, -1, :] = imgs[:, :, 0] = imgs[:, :, -1] = 0
+    ranges = lolahip.compute_line_ranges(1024, geom.width, geom.height, geom.matrices, 16,
+                                         device=True)
+    kw = dict(precision=prec, kernel="fused", views_per_pass=3, run_length=32)
+    staged = lolahip.backproject_views(1024, geom.matrices, imgs, ranges, staging=True, **kw)
+    gathered = lolahip.backproject_views(1024, geom.matrices, imgs, ranges, staging=False, **kw)
+    assert np.abs(staged).max() > 0
+    assert np.array_equal(staged.view(np.uint32), gathered.view(np.uint32))
+
+
 @pytest.mark.parametrize("kernel,staging,prec", [("fused", True, "f32"), ("fused", False, "f32"),
                                                 ("plain", True, "f32"), ("fused", True, "bf16")])
 def test_z_slab_equals_full_volume_slice(kernel, staging, prec):
