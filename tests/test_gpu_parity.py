@@ -89,6 +89,94 @@ def test_zero_images_give_zero_volume(golden_border):
     assert not _bp(g).any()
 
 
+# Closed forms of the unclipped backprojection (LolaBunny.c:11-30,44-75),
+# evaluated in FP64 numpy from the projection matrices alone.  Values 1.5, 1
+# and 3 are exact in every storage precision, so one tolerance holds for all.
+CLOSED_FORM_PATHS = [("plain", 1, "f32", False), ("fused", 4, "f32", True),
+                     ("fused", 4, "f32", False), ("fused", 4, "bf16", True),
+                     ("fused", 4, "f16", True), ("fused", 4, "f16", False)]
+
+
+@pytest.fixture(scope="module")
+def closed_geom():
+    return ScanGeometry(n_views=4, width=104, height=80, pixel=3.6, seed=3).build()
+
+
+def _project_grid(a):
+    """(u, v, w) of every voxel centre under the 3x4 column-major matrix a,
+    in FP64, indexed [z, y, x]."""
+    c = volume_origin(L) + np.arange(L) * voxel_size(L)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    P = np.asarray(a, np.float64).reshape(4, 3).T
+    w = P[2, 0] * x + P[2, 1] * y + P[2, 2] * z + P[2, 3]
+    return ((P[0, 0] * x + P[0, 1] * y + P[0, 2] * z + P[0, 3]) / w,
+            (P[1, 0] * x + P[1, 1] * y + P[1, 2] * z + P[1, 3]) / w, w)
+
+
+def _bp_closed(m, img, kernel, vpp, prec, staging):
+    return lolahip.backproject_views(L, m, img, None, kernel=kernel, views_per_pass=vpp,
+                                     precision=prec, staging=staging)
+
+
+@pytest.mark.parametrize("kernel,vpp,prec,staging", CLOSED_FORM_PATHS)
+def test_constant_image_gives_sum_of_inverse_square_depth(closed_geom, kernel, vpp, prec,
+                                                          staging):
+    """A constant c inside a zero border gives f = sum_n c / w_n^2 wherever all
+    four taps of every view lie inside the constant region."""
+    g = closed_geom
+    W, H = g.width, g.height
+    img = np.zeros((g.n_views, H, W), np.float32)
+    img[:, 1:H - 1, 1:W - 1] = 1.5
+    vol = _bp_closed(g.matrices, img, kernel, vpp, prec, staging)
+    expect = np.zeros((L, L, L))
+    inside = np.ones((L, L, L), bool)
+    for n in range(g.n_views):
+        u, v, w = _project_grid(g.matrices[n])
+        expect += 1.5 / w ** 2
+        # a margin of 0.01 px keeps FP32 coordinates on the same side
+        inside &= (u >= 1.01) & (u < W - 3.01) & (v >= 1.01) & (v < H - 3.01)
+    assert inside.sum() > 100000
+    np.testing.assert_allclose(vol[inside], expect[inside], rtol=1e-5)
+
+
+@pytest.mark.parametrize("kernel,vpp,prec,staging", CLOSED_FORM_PATHS)
+def test_single_pixel_gives_tent_weight(closed_geom, kernel, vpp, prec, staging):
+    """One pixel of value 1 at (i0, j0) gives each voxel the bilinear tent
+    weight of that pixel divided by w^2."""
+    g = closed_geom
+    a = g.matrices[1]
+    i0, j0 = 50, 37
+    img = np.zeros((1, g.height, g.width), np.float32)
+    img[0, j0, i0] = 1.0
+    vol = _bp_closed(a[None], img, kernel, vpp, prec, staging)
+    u, v, w = _project_grid(a)
+    expect = (np.clip(1 - np.abs(u - i0), 0, None) * np.clip(1 - np.abs(v - j0), 0, None)
+              / w ** 2)
+    assert (expect > 0).sum() > 500
+    # FP32 detector coordinates carry about 1e-5 px of error
+    np.testing.assert_allclose(vol, expect, rtol=0, atol=1e-4 * expect.max())
+
+
+@pytest.mark.parametrize("kernel,vpp,prec,staging", CLOSED_FORM_PATHS)
+def test_truncation_extrapolates_left_of_column_zero(closed_geom, kernel, vpp, prec, staging):
+    """u in (-1, 0) truncates to column 0: weight 1-u > 1 on column 0 and
+    u < 0 on column 1 (LolaBunny.c:19-30), so columns holding 1 and 3 give 1 + 2u."""
+    g = closed_geom
+    a = g.matrices[0]
+    u, v, w = _project_grid(a)
+    # the value jumps at u = -1 (column -1 is outside the image): keep clear of it
+    sel = (u > -0.99) & (u < -0.01) & (v > 5) & (v < g.height - 6)
+    assert sel.sum() > 100
+    img = np.zeros((1, g.height, g.width), np.float32)
+    img[0, :, 0] = 1.0
+    img[0, :, 1] = 3.0
+    vol = _bp_closed(a[None], img, kernel, vpp, prec, staging)
+    expect = (1 + 2 * u) / w ** 2
+    np.testing.assert_allclose(vol[sel], expect[sel], rtol=0,
+                               atol=1e-4 * np.abs(expect[sel]).max())
+    assert (expect[sel] < 0).any() and (expect[sel] > 0).any()
+
+
 def test_device_line_ranges_byte_equal_reference(golden):
     N, H, W = golden["images"].shape
     for vs in (16, 8):
