@@ -147,33 +147,48 @@ __device__ __forceinline__ float f16_hi(uint32_t q)
   return __half2float(__ushort_as_half((unsigned short)(q >> 16)));
 }
 
+// Horizontal lerp of one "pair" dword (p0, p1) = (p(i,j), p(i+1,j)) at
+// fraction ax, the same operations on both tap paths (LDS and L2).
+//  F16: (1-ax) p0 + ax p1, the FP16 operands widened inside the FMAs.
+//  BF16: p0 + ax (p1 - p0) in FP32 after widening (exact: shift and mask).
+template <int PREC>
+__device__ __forceinline__ float lerp_pair(uint32_t q, float ax)
+{
+  if constexpr (PREC == LH_PREC_F16) {
+    return fmaf(ax, f16_hi(q), fmaf(-ax, f16_lo(q), f16_lo(q)));
+  } else {
+    const float p0 = bf16_lo(q);
+    return fmaf(ax, bf16_hi(q) - p0, p0);
+  }
+}
+
 // The four taps of the bilinear stencil at integer corner (iix, iiy), both
 // already clamped to [-2, W] x [-2, H].
-//  FP32: row-major padded image, four 4-byte loads (rows iiy and iiy+1).
-//  BF16/F16: "pair" layout, element (i,j) holds p(i,j), p(i+1,j) as 2 halves
-//  in one dword: two 4-byte loads (rows iiy and iiy+1).
-template <int PREC, class Args>
+//  fetch_taps (FP32): row-major padded image, four 4-byte loads (rows iiy
+//  and iiy+1).
+//  fetch_pairs (BF16/F16): "pair" layout, element (i,j) holds p(i,j),
+//  p(i+1,j) as 2 halves in one dword: two 4-byte loads (rows iiy and iiy+1),
+//  returned packed.
+template <class Args>
 __device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, Args &A,
     int iix, int iiy, float &p00, float &p10, float &p01, float &p11)
 {
-  if constexpr (PREC == LH_PREC_F32) {
-    const int pitch    = (int)(A.rowBytes >> 2);
-    const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
-    p00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-    p10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0));
-    p01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0));
-    p11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, (int)A.rowBytes, 0));
-  } else {
-    const int pitch    = (int)(A.rowBytes >> 2);
-    const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
-    const uint32_t q0 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
-    const uint32_t q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0);
-    if constexpr (PREC == LH_PREC_BF16) {
-      p00 = bf16_lo(q0); p10 = bf16_hi(q0); p01 = bf16_lo(q1); p11 = bf16_hi(q1);
-    } else {
-      p00 = f16_lo(q0); p10 = f16_hi(q0); p01 = f16_lo(q1); p11 = f16_hi(q1);
-    }
-  }
+  const int pitch    = (int)(A.rowBytes >> 2);
+  const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
+  p00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+  p10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0));
+  p01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0));
+  p11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, (int)A.rowBytes, 0));
+}
+
+template <class Args>
+__device__ __forceinline__ void fetch_pairs(__amdgpu_buffer_rsrc_t rs, Args &A,
+    int iix, int iiy, uint32_t &q0, uint32_t &q1)
+{
+  const int pitch    = (int)(A.rowBytes >> 2);
+  const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
+  q0 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+  q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0);
 }
 
 // One voxel-view update: returns val / w^2 for the voxel at homogeneous
@@ -189,10 +204,18 @@ __device__ __forceinline__ float voxel_update(__amdgpu_buffer_rsrc_t rs, Args &A
   const float fy = __builtin_truncf(iy);
   const float ax = ix - fx;
   const float ay = iy - fy;
-  float p00, p10, p01, p11;
-  fetch_taps<PREC>(rs, A, (int)fx, (int)fy, p00, p10, p01, p11);
-  const float b   = fmaf(ax, p10 - p00, p00);
-  const float t   = fmaf(ax, p11 - p01, p01);
+  float b, t;
+  if constexpr (PREC == LH_PREC_F32) {
+    float p00, p10, p01, p11;
+    fetch_taps(rs, A, (int)fx, (int)fy, p00, p10, p01, p11);
+    b = fmaf(ax, p10 - p00, p00);
+    t = fmaf(ax, p11 - p01, p01);
+  } else {
+    uint32_t q0, q1;
+    fetch_pairs(rs, A, (int)fx, (int)fy, q0, q1);
+    b = lerp_pair<PREC>(q0, ax);
+    t = lerp_pair<PREC>(q1, ax);
+  }
   const float val = fmaf(ay, t - b, b);
   return val * (rw * rw);
 }
@@ -486,17 +509,8 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
   } else {
     // one dword per pixel: the horizontal pair (p(i,j), p(i+1,j))
     const uint32_t *qq = reinterpret_cast<const uint32_t *>(q);
-    const uint32_t q0 = qq[0], q1 = qq[kPitch];
-    if constexpr (PREC == LH_PREC_F16) {
-      // (1-ax) p0 + ax p1 with the FP16 operands widened inside the FMA
-      b = fmaf(ax, f16_hi(q0), fmaf(-ax, f16_lo(q0), f16_lo(q0)));
-      t = fmaf(ax, f16_hi(q1), fmaf(-ax, f16_lo(q1), f16_lo(q1)));
-    } else {
-      const float p00 = bf16_lo(q0), p10 = bf16_hi(q0);
-      const float p01 = bf16_lo(q1), p11 = bf16_hi(q1);
-      b = fmaf(ax, p10 - p00, p00);
-      t = fmaf(ax, p11 - p01, p01);
-    }
+    b = lerp_pair<PREC>(qq[0], ax);
+    t = lerp_pair<PREC>(qq[kPitch], ax);
   }
   return fmaf(ay, t - b, b) * (rw * rw);
 }

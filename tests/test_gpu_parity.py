@@ -289,7 +289,8 @@ def _full_detector_case(L2, n_views, seed, noise):
 
 
 @pytest.mark.parametrize("noise", [False, True])
-@pytest.mark.parametrize("prec,run", [("f32", 16), ("f32", 32), ("bf16", 16), ("bf16", 32)])
+@pytest.mark.parametrize("prec,run", [("f32", 16), ("f32", 32), ("bf16", 16), ("bf16", 32),
+                                      ("f16", 16), ("f16", 32)])
 def test_staged_equals_l2_gathers_bitwise(prec, run, noise):
     """The fused kernel with LDS-staged footprints (clamp-free interior
     update, clamped update at the detector edge, lanes outside their range
@@ -306,7 +307,7 @@ def test_staged_equals_l2_gathers_bitwise(prec, run, noise):
     assert diff == 0
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("prec", ["f32", "bf16", "f16"])
 def test_l1024_compact_run32_staged_equals_l2_gathers_bitwise(prec):
     """At L=1024 a 32-voxel run's tile spans 8 mm along x, so the staged
     kernel runs in its 128-VGPR form (4 workgroups per CU); it must still
