@@ -147,28 +147,26 @@ __device__ __forceinline__ float f16_hi(uint32_t q)
   return __half2float(__ushort_as_half((unsigned short)(q >> 16)));
 }
 
-// Horizontal lerp of one "pair" dword (p0, p1) = (p(i,j), p(i+1,j)) at
-// fraction ax, the same operations on both tap paths (LDS and L2).
-//  F16: (1-ax) p0 + ax p1, the FP16 operands widened inside the FMAs.
-//  BF16: p0 + ax (p1 - p0) in FP32 after widening (exact: shift and mask).
+// Horizontal lerp of one "pair" dword (p0, d) = (p(i,j), p(i+1,j) - p(i,j))
+// at fraction ax: p0 + ax d, one FMA with both halves widened (FP16: inside
+// the FMA; BF16: shift and mask).  The same operations on both tap paths (LDS
+// and L2).
 template <int PREC>
 __device__ __forceinline__ float lerp_pair(uint32_t q, float ax)
 {
-  if constexpr (PREC == LH_PREC_F16) {
-    return fmaf(ax, f16_hi(q), fmaf(-ax, f16_lo(q), f16_lo(q)));
-  } else {
-    const float p0 = bf16_lo(q);
-    return fmaf(ax, bf16_hi(q) - p0, p0);
-  }
+  if constexpr (PREC == LH_PREC_F16)
+    return fmaf(ax, f16_hi(q), f16_lo(q));
+  else
+    return fmaf(ax, bf16_hi(q), bf16_lo(q));
 }
 
 // The four taps of the bilinear stencil at integer corner (iix, iiy), both
 // already clamped to [-2, W] x [-2, H].
 //  fetch_taps (FP32): row-major padded image, four 4-byte loads (rows iiy
 //  and iiy+1).
-//  fetch_pairs (BF16/F16): "pair" layout, element (i,j) holds p(i,j),
-//  p(i+1,j) as 2 halves in one dword: two 4-byte loads (rows iiy and iiy+1),
-//  returned packed.
+//  fetch_pairs (BF16/F16): "pair" layout, element (i,j) holds p(i,j) and
+//  p(i+1,j) - p(i,j) as 2 halves in one dword: two 4-byte loads (rows iiy and
+//  iiy+1), returned packed.
 template <class Args>
 __device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, Args &A,
     int iix, int iiy, float &p00, float &p10, float &p01, float &p11)
@@ -507,7 +505,7 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
     b = fmaf(ax, p10 - p00, p00);
     t = fmaf(ax, p11 - p01, p01);
   } else {
-    // one dword per pixel: the horizontal pair (p(i,j), p(i+1,j))
+    // one dword per pixel: p(i,j) and the forward difference to p(i+1,j)
     const uint32_t *qq = reinterpret_cast<const uint32_t *>(q);
     b = lerp_pair<PREC>(qq[0], ax);
     t = lerp_pair<PREC>(qq[kPitch], ax);
@@ -760,8 +758,9 @@ __global__ void maxabs_kernel(const float *__restrict__ img, uint32_t n, uint32_
     atomicMax(out, m);
 }
 
-// Storage scale s = 4^m: FP16 keeps max|p|/s in (8192, 32768]; BF16 and FP32
-// use s = 1.  A power of four keeps k = s^-1/2 a power of two, so scaling the
+// Storage scale s = 4^m: FP16 keeps max|p|/s in (4096, 16384], so forward
+// differences (at most 2 max|p|/s) stay below FP16's largest finite 65504;
+// BF16 and FP32 use s = 1.  A power of four keeps k = s^-1/2 a power of two, so scaling the
 // matrix by k is exact and 1/w^2 picks up exactly the factor s.
 __device__ __forceinline__ int storage_exponent(int prec, uint32_t maxbits)
 {
@@ -770,8 +769,8 @@ __device__ __forceinline__ int storage_exponent(int prec, uint32_t maxbits)
   const float mx = __uint_as_float(maxbits);
   int m = 0;
   float s = 1.0f;
-  while (mx / s > 32768.0f && m < 60) { s *= 4.0f; ++m; }
-  while (mx / s <= 8192.0f && m > -60) { s *= 0.25f; --m; }
+  while (mx / s > 16384.0f && m < 60) { s *= 4.0f; ++m; }
+  while (mx / s <= 4096.0f && m > -60) { s *= 0.25f; --m; }
   return m;
 }
 
@@ -791,7 +790,9 @@ __device__ __forceinline__ uint16_t to_f16(float f)
 
 // Writes the bordered slot image from a W x H float image.  FP32: element
 // (i,j), i in [-2, W+1], j in [-2, H+1]; halves: pair element (i,j), i in
-// [-2, W], j in [-2, H+1].  Block (0,0) thread 0 also applies the storage scale
+// [-2, W], j in [-2, H+1], holding p(i,j) rounded and the difference
+// p(i+1,j) - (the rounded p(i,j)) rounded, so p0 + d reproduces p(i+1,j) to
+// the difference's own rounding.  Block (0,0) thread 0 also applies the storage scale
 // to the slot's view parameters.
 template <int PREC>
 __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
@@ -824,11 +825,11 @@ __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_
   } else {
     if (ei >= W + 3)
       return;
-    uint16_t h[2];
-    const float v[2] = { pix(i, j) * is, pix(i + 1, j) * is };
-    for (int c = 0; c < 2; ++c)
-      h[c] = PREC == LH_PREC_BF16 ? to_bf16(v[c]) : to_f16(v[c]);
-    const uint32_t q = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    const float v0 = pix(i, j) * is, v1 = pix(i + 1, j) * is;
+    const uint16_t h0 = PREC == LH_PREC_BF16 ? to_bf16(v0) : to_f16(v0);
+    const float p0 = PREC == LH_PREC_BF16 ? bf16_lo(h0) : f16_lo(h0);
+    const uint16_t h1 = PREC == LH_PREC_BF16 ? to_bf16(v1 - p0) : to_f16(v1 - p0);
+    const uint32_t q = (uint32_t)h0 | ((uint32_t)h1 << 16);
     reinterpret_cast<uint32_t *>(slot + (size_t)ej * rowBytes)[ei] = q;
   }
 }
