@@ -11,11 +11,19 @@ phantom, generated on the GPU).  Reported:
   variants     GUPS of plain FP32 (one view per launch), fused FP32, fused
                BF16 and FP16 projection storage, with their error against the
                plain FP32 volume
-  roofline     the headline kernel: effective voxel-view updates per launch x
-               (4*s_p + 8/b) bytes / mean launch time (HIP events on the
-               launch stream), against the 8 TB/s HBM peak
+  h2d_inclusive the same reconstruction from views in pinned host memory: H2D
+               copies, staging and graph-captured fused batches pipelined on
+               three streams, timed from the first copy to the last kernel
+               (the per-view copy LolaOPT's timed iterate makes,
+               LolaOPT.c:111-117), with the copies alone timed beside it
+  roofline     the headline kernel against its binding ceiling, VALU issue:
+               VALU lane-instructions per launch (rocprofv3 SQ_INSTS_VALU,
+               profiles/valu.json) / mean launch time (HIP events on the
+               launch stream) against CUs x 64 lanes x clock; beside it the
+               HBM traffic and compulsory bytes against the 8 TB/s peak
   cpu_baseline the reference's own LolaASM (and LolaOPT) from oracle/_ref,
-               run by the reference runner on this host on a sample of views
+               run by the reference runner on this host on a sample of views,
+               and the same runner driving LolaHIP (-m LolaHIP -b 16)
   sparse_cnn   the second half of the metric, at L=256 (R_L=1 mm): volumes/s
                of the sparse-view + U-Net pipeline (124 of 496 views, fused
                FP32 backprojection, U-Net, held-out-view residual check with
@@ -55,14 +63,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_context(args, geom, phantom, ranges, kernel, prec, vpp):
+def make_context(args, geom, phantom, ranges, kernel, prec, vpp, host_out=None):
     ctx = lolahip.Context(args.L, geom.width, geom.height, geom.n_views, n_slots=geom.n_views,
                           precision=prec, kernel=kernel, views_per_pass=vpp,
                           run_length=args.run)
     if args.slab:
         ctx.set_slab(*args.slab)
     ctx.set_ranges(ranges)
-    generate_into(ctx, geom, phantom)
+    generate_into(ctx, geom, phantom, out=host_out)
     ctx.synchronize()
     tuned = None
     if kernel == "fused" and not args.no_tune:
@@ -106,9 +114,71 @@ def time_variant(args, ctx, n_views):
     return ms, ms_launch, launches, vol
 
 
+def host_pipeline(args, geom, ranges, pinned, shape, prec, resident_vol):
+    """The reconstruction as a caller holding the views in host RAM runs it
+    (lhTimedReconstructHost): all views in pinned host memory, per step the
+    H2D copies on the upload stream, staging on its own stream and the
+    graph-captured fused batches on the compute stream, double-buffered over
+    two slot sets of the tuned views-per-pass, timed with HIP events from the
+    first copy to the last kernel.  Also times the copies alone (the PCIe
+    leg) and checks the pipeline's volume against the resident path's bit for
+    bit."""
+    vpp, run, staging = shape["views_per_pass"], shape["run_length"], shape["staging"]
+    N = geom.n_views
+    ids = np.arange(1, N + 1, dtype=np.int32)
+    with lolahip.Context(args.L, geom.width, geom.height, N, n_slots=2 * vpp, precision=prec,
+                         kernel="fused", views_per_pass=vpp, run_length=run,
+                         staging=staging) as ctx:
+        if args.slab:
+            ctx.set_slab(*args.slab)
+        ctx.set_ranges(ranges)
+        ctx.set_graphs(True)
+        ctx.reconstruct_host(pinned, geom.matrices, ids)     # captures the batch graphs
+        ctx.zero_volume()
+        ctx.reconstruct_host(pinned, geom.matrices, ids)
+        vol = ctx.get_volume()
+        same = bool(np.array_equal(vol.view(np.uint32), resident_vol.view(np.uint32)))
+        del vol
+        if args.warmup:
+            ctx.timed_reconstruct_host(pinned, geom.matrices, ids, args.warmup)
+        t0 = time.perf_counter()
+        ms = ctx.timed_reconstruct_host(pinned, geom.matrices, ids, args.steps)
+        wall = (time.perf_counter() - t0) / args.steps * 1e3
+        ms_h2d = ctx.timed_upload_host(pinned, max(1, min(args.steps, 3)))
+    depth = args.slab[1] - args.slab[0] if args.slab else args.L
+    nbytes = pinned.nbytes
+    out = {"ms_per_step": round(ms, 3), "gups": round(depth * args.L ** 2 * N / ms / 1e6, 2),
+           "host_wall_ms_per_step": round(wall, 3),
+           "h2d_only_ms": round(ms_h2d, 3), "h2d_gb_per_s": round(nbytes / ms_h2d / 1e6, 2),
+           "h2d_bytes_per_step": nbytes, "equal_to_resident_bitwise": same,
+           "shape": {**shape, "slots": 2 * vpp, "graphs": True},
+           "how": "lhTimedReconstructHost: views in pinned host memory (lhHostAlloc); H2D "
+                  "copies, staging and graph-captured fused batches on three streams, "
+                  "double-buffered slot sets; HIP events from the first copy to the last "
+                  "kernel"}
+    log(f"host pipeline ({prec}): {ms:.2f} ms/step = {out['gups']} GUPS; H2D alone "
+        f"{ms_h2d:.2f} ms ({out['h2d_gb_per_s']} GB/s); equal to resident: {same}")
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, geom, phantom, ranges):
     """The reference runner (oracle/_ref) on this host: LolaASM and LolaOPT on
-    a sample of the same views, with the same clip ranges."""
+    a sample of the same views, with the same clip ranges; then the same
+    runner driving LolaHIP (-m LolaHIP -b 16) over the same file, the
+    runner-driven GPU figure (its timer covers each iterate call, uploads
+    included; the reads between calls are untimed, rabbitRunner.c:256-285)."""
     from oracle import oracle as orc
     isa = orc.host_isa()
     runner = orc.ref_runner(isa)
@@ -147,18 +217,39 @@ def cpu_baseline(args, geom, phantom, ranges):
             total = float(re.search(r"Total: ([0-9.eE+-]+) s", p.stdout).group(1))
             out[(alg, b)] = {"seconds": total, "views": n,
                              "gups": args.L ** 3 * n / total / 1e9}
+        hip_runner = orc.ref_runner(isa, lolahip=True)
+        if hip_runner is not None and n % 16 == 0 and not args.slab:
+            log(f"runner-driven LolaHIP: -m LolaHIP -b 16 on {n} views")
+            p = subprocess.run([hip_runner, "-i", "in.rct", "-a", "geo.rct", "-m", "LolaHIP",
+                                "-s", str(args.L), "-b", "16", "-C", "clip.rct"], cwd=d,
+                               env=env, capture_output=True, text=True,
+                               timeout=900)
+            if p.returncode == 0:
+                total = float(re.search(r"Total: ([0-9.eE+-]+) s", p.stdout).group(1))
+                out["LolaHIP"] = {"seconds": total, "views": n,
+                                  "gups": args.L ** 3 * n / total / 1e9}
+            else:
+                log(p.stdout[-2000:] + p.stderr[-2000:])
     asm = out[("LolaASM", 1)]
     opt = out[("LolaOPT", 1)]
     asm16 = out.get(("LolaASM", 16))
+    hip = out.get("LolaHIP")
     return {"value": round(asm["gups"], 3), "unit": "GUPS", "cores": cores,
-            "kind": "reference",
+            "kind": "reference", "cpu_model": cpu_model(),
             "sample": f"LolaASM ({isa}, VECTORSIZE {16 if isa == 'avx512' else 8}) over the "
                       f"first {asm['views']} of the {geom.n_views} views at L={args.L}, -b 1, "
                       f"reference runner Total {asm['seconds']:.3f} s",
             "lolaopt_gups": round(opt["gups"], 3),
             "lolaopt_sample": f"first {opt['views']} views, -b 1, Total "
                               f"{opt['seconds']:.3f} s",
-            "lolaasm_b16_gups": round(asm16["gups"], 3) if asm16 else None}
+            "lolaasm_b16_gups": round(asm16["gups"], 3) if asm16 else None,
+            "runner_driven_lolahip": None if hip is None else {
+                "gups": round(hip["gups"], 2), "total_s": round(hip["seconds"], 4),
+                "views": hip["views"],
+                "how": "reference rabbitRunner -m LolaHIP -b 16 (module defaults: FP32, 16 "
+                       "views per pass), Total of the timed iterate calls; every call that "
+                       "launches a batch returns after it finished; uploads from the "
+                       "runner's pageable buffers included"}}
 
 
 def sparse_cnn(args):
@@ -184,6 +275,15 @@ def sparse_cnn(args):
 
     ms_full, _ = timed(lambda: scan.full_view(*rec.full_kernel), args.steps)
     full = scan.ctx.get_volume().astype(np.float64)
+    # the fastest full-view shape this library has at L=256 (fused; most
+    # footprints there exceed the LDS window, so 32-voxel runs on L2 gathers
+    # compete with the staged form)
+    fused_ms = {}
+    for shape in (("fused", 64, 32, True), ("fused", 32, 32, True), ("fused", 64, 16, True),
+                  ("fused", 64, 32, False)):
+        fused_ms[shape], _ = timed(lambda: scan.full_view(*shape), args.steps)
+    best_fused = min(fused_ms, key=fused_ms.get)
+    ms_full_fused = fused_ms[best_fused]
     ms_pipe, res = timed(lambda: rec.reconstruct(scan), args.steps)
     vol = rec.download(res, L)
     scan.sparse_view(*rec.sparse_kernel)
@@ -195,6 +295,10 @@ def sparse_cnn(args):
         "volumes_per_s_full_plain": round(1e3 / ms_full, 3),
         "volumes_per_s_sparse_cnn": round(1e3 / ms_pipe, 3),
         "speedup": round(ms_full / ms_pipe, 3),
+        "ms_full_fused": round(ms_full_fused, 3),
+        "full_fused_shape": {"kernel": best_fused[0], "views_per_pass": best_fused[1],
+                             "run_length": best_fused[2], "staging": best_fused[3]},
+        "speedup_vs_fused_full": round(ms_full_fused / ms_pipe, 3),
         "ms_full_plain": round(ms_full, 3),
         "ms_sparse_cnn": round(ms_pipe, 3),
         "ms_stages": {k: round(v, 3) for k, v in res["ms"].items()},
@@ -211,17 +315,26 @@ def sparse_cnn(args):
                            "unit": "TFLOP/s", "frac": round(tf / MFMA_BF16_PEAK, 4),
                            "flops_per_volume": flops}
     log(f"sparse_cnn: full plain {ms_full:.2f} ms, sparse+CNN {ms_pipe:.2f} ms "
-        f"({out['speedup']}x), rmse {out['rmse_sparse_cnn_vs_full']:.4g} "
+        f"({out['speedup']}x; vs fused full {ms_full_fused:.2f} ms: "
+        f"{out['speedup_vs_fused_full']}x), rmse {out['rmse_sparse_cnn_vs_full']:.4g} "
         f"(sparse only {out['rmse_sparse_only_vs_full']:.4g}), fallback {res['fallback']}")
     scan.close()
     return out
 
 
-def measured_traffic(variant, shape, L):
-    """HBM bytes per launch of the headline kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), already scaled by the
-    counters' reading of a 1 GiB copy; None when no pass matches this shape."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+def source_sha():
+    """Short hash of the kernel source, recorded with each profile so a
+    counter taken on another build is flagged as stale."""
+    import hashlib
+    with open(os.path.join(ROOT, "rabbitct_amd", "csrc", "lh_device.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:12]
+
+
+def profile_record(name, variant, shape, L):
+    """Record of the headline kernel in a committed profile table
+    (profiles/<name>.json, written from rocprofv3 --pmc passes); None when no
+    pass matches this shape.  ``stale`` says the kernel source changed since."""
+    path = os.path.join(ROOT, "profiles", f"{name}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -229,7 +342,64 @@ def measured_traffic(variant, shape, L):
     key = (f"{variant} L={L} vpp={shape['views_per_pass']} R={shape['run_length']} "
            f"st={int(shape['staging'])}")
     rec = data.get(key)
-    return None if rec is None else rec["bytes_per_launch"]
+    if rec is None:
+        return None
+    return {**rec, "stale": rec.get("source_sha") != source_sha()}
+
+
+def roofline(args, ranges_eff, head, lat, slot_bytes):
+    """The headline kernel against the ceiling that binds it.
+
+    The update is a gather-interpolate-accumulate whose taps come from LDS:
+    the kernel is bound by VALU issue, not by HBM.  achieved = VALU
+    lane-instructions per launch (SQ_INSTS_VALU x 64 from the committed
+    rocprofv3 pass, profiles/valu.json, deterministic for a given build,
+    shape and input) / the live mean launch time; peak = compute units x 64
+    lanes x engine clock (one wave64 VALU instruction per CU per clock, packed
+    FP32 off), from the device.  Beside it: the HBM traffic (FETCH_SIZE +
+    WRITE_SIZE, profiles/traffic.json) and the compulsory HBM bytes (volume
+    read + write, the launch's slots once, their line ranges) over the launch
+    time against the 8 TB/s peak, and the 4 s_p + 8/b tap-byte formula's
+    ratio to the HBM peak (tap-byte efficiency: taps served from LDS count,
+    so it exceeds 1)."""
+    ms_launch, launches, eff, vpp, prec = lat
+    shape = head["shape"]
+    per_launch = eff / launches
+    t = ms_launch * 1e-3
+    info = lolahip.device_info(0)
+    peak = info["compute_units"] * 64 * info["clock_mhz"] * 1e6
+    valu = profile_record("valu", args.headline, shape, args.L)
+    traffic = profile_record("traffic", args.headline, shape, args.L)
+    depth = args.slab[1] - args.slab[0] if args.slab else args.L
+    compulsory = (8.0 * depth * args.L ** 2 + vpp * slot_bytes + vpp * 4.0 * depth * args.L)
+    s_p = 4 if prec == "f32" else 2
+    tap_bytes = 4 * s_p + 8.0 / vpp
+    out = {"bound": "valu", "unit": "T VALU lane-instructions/s",
+           "peak": round(peak / 1e12, 2),
+           "peak_how": f"{info['compute_units']} CUs x 64 lanes x {info['clock_mhz']:.0f} MHz",
+           "achieved": None, "frac": None, "traffic": None,
+           "updates_per_launch": int(per_launch), "ms_per_launch": round(ms_launch, 4)}
+    if valu is not None:
+        lane_instr = valu["valu_insts_per_launch"] * 64.0
+        per_update = lane_instr / valu["effective_updates_per_launch"]
+        ach = per_update * per_launch / t
+        out.update({"achieved": round(ach / 1e12, 2), "frac": round(ach / peak, 4),
+                    "valu_lane_instr_per_update": round(per_update, 2),
+                    "valu_source": "profiles/valu.json (rocprofv3 --pmc SQ_INSTS_VALU)",
+                    "valu_stale": valu["stale"]})
+    if traffic is not None:
+        out.update({"traffic": traffic["bytes_per_launch"],
+                    "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, "
+                                      "WRITE_SIZE, scaled by a 1 GiB copy)",
+                    "traffic_stale": traffic["stale"],
+                    "traffic_frac_of_hbm_peak": round(traffic["bytes_per_launch"] / t / HBM_PEAK,
+                                                      4)})
+    out["hbm"] = {"compulsory_bytes_per_launch": int(compulsory),
+                  "achieved_gb_per_s": round(compulsory / t / 1e9, 1),
+                  "frac": round(compulsory / t / HBM_PEAK, 4)}
+    out["tap_byte_efficiency"] = {"bytes_per_update": tap_bytes,
+                                  "value": round(per_launch * tap_bytes / t / HBM_PEAK, 4)}
+    return out
 
 
 def main():
@@ -252,6 +422,8 @@ def main():
     ap.add_argument("--tune-views", type=int, default=128)
     ap.add_argument("--ellipsoids", type=int, default=10,
                     help="phantom ellipsoids (30 for the L=1024 configuration)")
+    ap.add_argument("--no-h2d", action="store_true",
+                    help="skip the host-memory pipeline (H2D copies inside the timed step)")
     ap.add_argument("--no-sparse-cnn", action="store_true",
                     help="skip the sparse-view + CNN volumes/s measurement at L=256")
     ap.add_argument("--slab", default="",
@@ -275,11 +447,17 @@ def main():
     variants = args.variants.split(",")
     if args.headline not in variants:
         variants.insert(0, args.headline)
+    # the headline variant's projections are also kept in pinned host memory
+    # for the host-memory pipeline
+    pinned = None if args.no_h2d else lolahip.PinnedArray((args.views, geom.height,
+                                                           geom.width))
     for var in variants:
         kernel, prec = var.split("-")
         vpp = 1 if kernel == "plain" else args.vpp
         t0 = time.time()
-        ctx, tuned = make_context(args, geom, phantom, ranges, kernel, prec, vpp)
+        ctx, tuned = make_context(args, geom, phantom, ranges, kernel, prec, vpp,
+                                  host_out=None if (pinned is None or var != args.headline)
+                                  else pinned.array)
         if tuned:
             vpp = tuned["best"]["views_per_pass"]
         log(f"{var}: context + {args.views} views generated (+ tuning) in "
@@ -312,12 +490,17 @@ def main():
         results[var]["max_err_vs_plain_f32"] = float(np.abs(d).max() / rng)
         results[var]["rmse_vs_plain_f32"] = float(np.sqrt((d.astype(np.float64) ** 2).mean()) / rng)
 
-    ms_launch, launches, eff, vpp, prec = lat[args.headline]
-    s_p = 4 if prec == "f32" else 2
-    bytes_per_update = 4 * s_p + 8.0 / vpp
-    achieved = (eff / launches) * bytes_per_update / (ms_launch * 1e-3)
     head = results[args.headline]
-    traffic = measured_traffic(args.headline, head["shape"], args.L)
+    vpp = lat[args.headline][3]
+    h_prec = lat[args.headline][4]
+    cols = geom.width + (4 if h_prec == "f32" else 3)
+    slot_bytes = -(-cols * 4 // 256) * 256 * (geom.height + 4)
+    h2d = None
+    if pinned is not None and args.headline.startswith("fused"):
+        h2d = host_pipeline(args, geom, ranges, pinned.array, head["shape"], h_prec,
+                            vols[args.headline])
+    if pinned is not None:
+        pinned.close()
     line = {
         "metric": METRIC,
         "value": head["gups"],
@@ -341,11 +524,8 @@ def main():
                    "shape": head["shape"]},
         "gups_effective": head["gups_effective"],
         "variants": results,
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "bytes_per_update": bytes_per_update,
-                     "updates_per_launch": int(eff / launches)},
+        "h2d_inclusive": h2d,
+        "roofline": roofline(args, None, head, lat[args.headline], slot_bytes),
         "gather_roof": gather_roof(head),
         "cpu_baseline": None,
         "sparse_cnn": None if args.no_sparse_cnn else sparse_cnn(args),

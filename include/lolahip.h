@@ -75,13 +75,21 @@ int lolaHipPrepare(RabbitCtGlobalData *rcgd);
 
 /* Replaces lolaOptBackprojection (LolaOPT.c:91-179): backprojects the
  * rcgd->numberOfProjections buffered views into the volume (+=).  Views are
- * collected into batches of LOLAHIP_VIEWS_PER_PASS; the batch holding view
- * id == N is finished before this call returns. */
+ * collected into batches of LOLAHIP_VIEWS_PER_PASS (a batch may span calls:
+ * 496 views admit only -b values that divide 496); a batch launches when it
+ * is full or holds view id == N, and a call that launched a batch returns
+ * only after that work has finished, so the runner's timer around the call
+ * (rabbitRunner.c:55-58,279) covers it. */
 int lolaHipBackprojection(RabbitCtGlobalData *rcgd);
 
 /* Replaces lolaOptFinish (LolaOPT.c:86-89): flushes pending views, copies the
  * volume into rcgd->volumeData and releases the GPU resources. */
 int lolaHipFinish(RabbitCtGlobalData *rcgd);
+
+/* 1 when the module has no GPU work in flight, 0 while some runs, -1 before
+ * prepare (lhQueryIdle on the module's context).  A check of the timing
+ * contract above; no counterpart in the reference. */
+int lolaHipQueryIdle(void);
 
 /* ---------------------------------------------------------------------------
  * Geometry analysis with plain pointers (analyseGeometry.h:12-13).
@@ -197,6 +205,52 @@ int lhTimedBackproject(LhContext *ctx, uint32_t count, int reps, float *msPerRep
     float *msPerLaunch, uint32_t *launchesPerRep);
 
 int lhSynchronize(LhContext *ctx);
+
+/* 1 when all work enqueued on the context's streams (compute, upload,
+ * staging) has completed, 0 while some is still running, a negative LH_E*
+ * code on error.  Does not wait.  No counterpart in the reference. */
+int lhQueryIdle(LhContext *ctx);
+
+/* Compute units and peak engine clock (kHz) of a device, for rate
+ * ceilings.  No counterpart in the reference. */
+int lhDeviceInfo(int device, int *computeUnits, int *clockKHz);
+
+/* Pinned (page-locked) host memory, so H2D copies from it run asynchronously
+ * at the DMA rate.  The platform refuses to pin memory that already exists,
+ * so callers that upload from host memory allocate it here.  No counterpart
+ * in the reference (memoryUtilsAllocate, memoryUtils.c:43-67, allocates
+ * pageable memory). */
+int lhHostAlloc(void **p, uint64_t bytes);
+int lhHostFree(void *p);
+
+/* The whole reconstruction from host memory, as a caller that holds the views
+ * in RAM runs it (the per-view copy inside LolaOPT's timed iterate,
+ * LolaOPT.c:111-117, plus the backprojection): views [0, count) of images
+ * (count x H x W floats, host; pinned memory from lhHostAlloc makes the
+ * copies asynchronous), matrices (count x 12) and ids (count, 1-based) are
+ * uploaded on the context's upload stream, staged into the slots on a staging
+ * stream and backprojected (+=) on the compute stream, in batches of
+ * viewsPerPass views (fused; 16 views per batch for plain launches) over
+ * nSlots / batch slot sets: batch k+1 crosses PCIe and is staged while batch k
+ * is backprojected.  With lhSetGraphs on, each batch replays a captured HIP
+ * graph.  Asynchronous: lhSynchronize waits for the end.  The host arrays must
+ * stay unchanged until then.  No counterpart in the reference. */
+int lhReconstructHost(LhContext *ctx, const float *images, const double *matrices,
+    const int *ids, uint32_t count);
+
+/* lhReconstructHost `reps` times, each from idle streams, timed with HIP
+ * events from before the first H2D copy to after the last backprojection:
+ * mean milliseconds per repetition. */
+int lhTimedReconstructHost(LhContext *ctx, const float *images, const double *matrices,
+    const int *ids, uint32_t count, int reps, float *msPerRep);
+
+/* The PCIe leg of lhReconstructHost alone: the H2D copies of views
+ * [0, count) of images (host, count x H x W floats) into the context's device
+ * upload buffers on the upload stream, back to back, `reps` times; mean
+ * milliseconds per repetition (HIP events).  No counterpart in the
+ * reference. */
+int lhTimedUploadHost(LhContext *ctx, const float *images, uint32_t count, int reps,
+    float *msPerRep);
 
 /* enable != 0: lhBackproject replays one captured HIP graph per batch (slot
  * range and launch shape), capturing it on first use. */

@@ -429,10 +429,11 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
 // Issues the direct-to-LDS loads of footprint `b` of the slot image into buf.
 // FP32: each buffer_load_dwordx4 ... lds moves two rows of 128 pixels (lane l
 // lands 16 bytes at l*16, so lanes 32-63 fill the next LDS row); wave w copies
-// row pairs 2w, 2w+8, ... (a half image holds one dword per element too: the
-// horizontal pair p(i,j), p(i+1,j)).  laneOff is the lane's byte offset within
-// a row pair, fixed per tile; the loop control is wave-uniform and only the
-// lower half-wave copies a last odd row.  Other layouts: one dword per lane,
+// row pairs 2w, 2w+8, ... (a half image holds one dword per element too:
+// p(i,j) and the forward difference p(i+1,j) - p(i,j)).  laneOff is the lane's
+// byte offset within a row pair, fixed per tile; the loop control is
+// wave-uniform and only the lower half-wave copies a last odd row.  Other
+// layouts: one dword per lane,
 // pixels [iu0 + c0, iu0 + c0 + 64) of a row per instruction, wave w copying
 // rows w, w+4, ....  Columns at or beyond fw are copied too and never
 // read; past the end of the slot the buffer bounds return zero.  Completion is
@@ -746,16 +747,37 @@ __global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
 
 // ---- projection staging -------------------------------------------------
 
-// max |p| over an image, as the bit pattern of a non-negative float
-__global__ void maxabs_kernel(const float *__restrict__ img, uint32_t n, uint32_t *out)
+// max |p| over an image, as the bit pattern of a non-negative float (the
+// order of non-negative floats is that of their bit patterns).  16-byte loads
+// when the image is 16-byte aligned; one atomic per workgroup, since atomics
+// on one address serialise in L2 (one per wave took ~49 us per 1248 x 960
+// image, one per workgroup of a 256-workgroup grid ~5 us).
+constexpr int kMaxAbsThreads = 256;
+__global__ __launch_bounds__(kMaxAbsThreads) void maxabs_kernel(const float *__restrict__ img,
+    uint32_t n, uint32_t *out)
 {
+  __shared__ uint32_t sMax[kMaxAbsThreads / kWave];
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  const uint32_t n4  = ((uintptr_t)img & 15) == 0 ? n / 4 : 0;
+  const float4 *i4   = reinterpret_cast<const float4 *>(img);
   uint32_t m = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+  for (uint32_t i = tid; i < n4; i += nt) {
+    const float4 q = i4[i];
+    m = max(m, max(max(__float_as_uint(fabsf(q.x)), __float_as_uint(fabsf(q.y))),
+                   max(__float_as_uint(fabsf(q.z)), __float_as_uint(fabsf(q.w)))));
+  }
+  for (uint32_t i = 4 * n4 + tid; i < n; i += nt)
     m = max(m, __float_as_uint(fabsf(img[i])));
   for (int o = 32; o > 0; o >>= 1)
     m = max(m, (uint32_t)__shfl_xor((int)m, o));
-  if ((threadIdx.x & 63) == 0)
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    sMax[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kMaxAbsThreads / kWave; ++w)
+      m = max(m, sMax[w]);
     atomicMax(out, m);
+  }
 }
 
 // Storage scale s = 4^m: FP16 keeps max|p|/s in (4096, 16384], so forward
@@ -798,7 +820,8 @@ template <int PREC>
 __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
     uint32_t rowBytes, const uint32_t *maxbits, ViewParams vp, ViewParams *out)
 {
-  const int m    = storage_exponent(PREC, *maxbits);
+  // only FP16 storage is scaled (maxabs_kernel filled *maxbits)
+  const int m    = PREC == LH_PREC_F16 ? storage_exponent(PREC, *maxbits) : 0;
   const float is = ldexpf(1.0f, -2 * m);
   const int ei   = blockIdx.x * blockDim.x + threadIdx.x; // element column
   const int ej   = blockIdx.y;                            // element row
@@ -926,6 +949,13 @@ struct LhContext {
   int nRing = 0, ringNext = 0, ringAcquired = -1;
   std::vector<float *> hRing, dRing;
   std::vector<hipEvent_t> copyDone, stageDone;
+  // host-memory reconstruction (lhReconstructHost): staging kernels run on
+  // stageStream, beside the backprojection of the previous batch; per slot
+  // set, an event after its last staging kernel and one after the
+  // backprojection that read it
+  hipStream_t stageStream = nullptr;
+  std::vector<hipEvent_t> setStaged, setDone;
+  std::vector<bool> setUsed;
   // captured backprojection batches, by (first slot, count, kernel, run, staging)
   bool graphs = false;
   std::map<std::tuple<uint32_t, uint32_t, int, int, int>, hipGraphExec_t> graphCache;
@@ -1065,6 +1095,7 @@ static int create_buffers(LhContext *c)
   // one ring entry per view of a register pass (2..32), so a whole batch can
   // be in flight while the previous one is backprojected
   LH_HIP(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
+  LH_HIP(hipStreamCreateWithFlags(&c->stageStream, hipStreamNonBlocking));
   c->nRing = (int)std::min<uint32_t>(32u, std::max<uint32_t>(2u, cfg->viewsPerPass));
   c->hRing.assign(c->nRing, nullptr);
   c->dRing.assign(c->nRing, nullptr);
@@ -1088,14 +1119,18 @@ int lhDestroy(LhContext *c)
 {
   if (!c)
     return LH_OK;
-  if (c->stream)
-    (void)hipStreamSynchronize(c->stream);
+  // every stream drains before the buffers its work reads or writes go away
+  for (hipStream_t st : { c->stream, c->upStream, c->stageStream })
+    if (st)
+      (void)hipStreamSynchronize(st);
   (void)hipFree(c->dVol);
   (void)hipFree(c->dRanges);
   (void)hipFree(c->dViews);
   (void)hipFree(c->dSlots);
-  if (c->upStream)
-    (void)hipStreamSynchronize(c->upStream);
+  for (hipEvent_t e : c->setStaged)
+    (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->setDone)
+    (void)hipEventDestroy(e);
   for (auto &kv : c->graphCache)
     (void)hipGraphExecDestroy(kv.second);
   (void)hipFree(c->dMax);
@@ -1111,6 +1146,8 @@ int lhDestroy(LhContext *c)
   }
   if (c->upStream)
     (void)hipStreamDestroy(c->upStream);
+  if (c->stageStream)
+    (void)hipStreamDestroy(c->stageStream);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1269,11 +1306,13 @@ int lhVolumeDevice(LhContext *c, float **dVol)
   return LH_OK;
 }
 
-// Enqueues on the context stream: max-abs of the W x H image src (device),
-// then its conversion into slot `slot` with the view parameters of `matrix`
-// (passed by value to the staging kernel, which writes them to the slot).
+// Enqueues on stream `st`: for FP16 storage the max-abs of the W x H image
+// src (device), which sets the slot's storage scale, then the conversion into
+// slot `slot` with the view parameters of `matrix` (passed by value to the
+// staging kernel, which writes them to the slot).  FP32 and BF16 are stored
+// unscaled and skip the max-abs pass.
 static int load_view_common(LhContext *c, uint32_t slot, int id, const double *matrix,
-    const float *src)
+    const float *src, hipStream_t st)
 {
   const LhConfig &g = c->cfg;
   if (slot >= g.nSlots || id < 1 || (uint32_t)id > g.nViews || !matrix)
@@ -1287,25 +1326,29 @@ static int load_view_common(LhContext *c, uint32_t slot, int id, const double *m
   vp.dw    = (float)(matrix[2] * (double)g.R_L);
   vp.id    = id;
   vp.scale = 1.0f;
-  LH_HIP(hipMemsetAsync(c->dMax + slot, 0, sizeof(uint32_t), c->stream));
-  const uint32_t n = g.W * g.H;
-  maxabs_kernel<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, c->stream>>>(src, n,
-      c->dMax + slot);
+  if (g.precision == LH_PREC_F16) {
+    LH_HIP(hipMemsetAsync(c->dMax + slot, 0, sizeof(uint32_t), st));
+    const uint32_t n = g.W * g.H;
+    // about four 16-byte loads per thread, at most 256 workgroups (atomics)
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(
+        (n / 4 + 4 * kMaxAbsThreads - 1) / (4 * kMaxAbsThreads), 256u));
+    maxabs_kernel<<<blocks, kMaxAbsThreads, 0, st>>>(src, n, c->dMax + slot);
+  }
   const int cols = g.precision == LH_PREC_F32 ? (int)g.W + 4 : (int)g.W + 3;
   const int rows = (int)g.H + 4;
   dim3 grid((cols + 255) / 256, rows);
   uint8_t *dst = c->dSlots + (uint64_t)slot * c->slotBytes;
   switch (g.precision) {
   case LH_PREC_F32:
-    stage_kernel<LH_PREC_F32><<<grid, 256, 0, c->stream>>>(src, g.W, g.H, dst, c->rowBytes,
+    stage_kernel<LH_PREC_F32><<<grid, 256, 0, st>>>(src, g.W, g.H, dst, c->rowBytes,
         c->dMax + slot, vp, c->dViews + slot);
     break;
   case LH_PREC_BF16:
-    stage_kernel<LH_PREC_BF16><<<grid, 256, 0, c->stream>>>(src, g.W, g.H, dst, c->rowBytes,
+    stage_kernel<LH_PREC_BF16><<<grid, 256, 0, st>>>(src, g.W, g.H, dst, c->rowBytes,
         c->dMax + slot, vp, c->dViews + slot);
     break;
   default:
-    stage_kernel<LH_PREC_F16><<<grid, 256, 0, c->stream>>>(src, g.W, g.H, dst, c->rowBytes,
+    stage_kernel<LH_PREC_F16><<<grid, 256, 0, st>>>(src, g.W, g.H, dst, c->rowBytes,
         c->dMax + slot, vp, c->dViews + slot);
     break;
   }
@@ -1357,7 +1400,7 @@ int lhCommitUpload(LhContext *c, uint32_t slot, int id, const double *matrix)
   LH_HIP(hipEventRecord(c->copyDone[k], c->upStream));
   // compute stream: stage once the copy has landed
   LH_HIP(hipStreamWaitEvent(c->stream, c->copyDone[k], 0));
-  const int rc = load_view_common(c, slot, id, matrix, c->dRing[k]);
+  const int rc = load_view_common(c, slot, id, matrix, c->dRing[k], c->stream);
   if (rc != LH_OK)
     return rc;
   LH_HIP(hipEventRecord(c->stageDone[k], c->stream));
@@ -1383,7 +1426,45 @@ int lhLoadViewDevice(LhContext *c, uint32_t slot, int id, const double *matrix,
   if (!c || !dImage)
     return fail(LH_E_ARG, "lhLoadViewDevice: null argument");
   LH_HIP(hipSetDevice(c->cfg.device));
-  return load_view_common(c, slot, id, matrix, dImage);
+  return load_view_common(c, slot, id, matrix, dImage, c->stream);
+}
+
+int lhQueryIdle(LhContext *c)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhQueryIdle: null context");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  for (hipStream_t st : { c->stream, c->upStream, c->stageStream }) {
+    const hipError_t e = hipStreamQuery(st);
+    if (e == hipErrorNotReady)
+      return 0;
+    if (e != hipSuccess)
+      return fail(LH_E_HIP, "lhQueryIdle: %s", hipGetErrorString(e));
+  }
+  return 1;
+}
+
+int lhDeviceInfo(int device, int *computeUnits, int *clockKHz)
+{
+  LH_HIP(hipDeviceGetAttribute(computeUnits, hipDeviceAttributeMultiprocessorCount, device));
+  LH_HIP(hipDeviceGetAttribute(clockKHz, hipDeviceAttributeClockRate, device));
+  return LH_OK;
+}
+
+int lhHostAlloc(void **p, uint64_t bytes)
+{
+  if (!p || bytes == 0)
+    return fail(LH_E_ARG, "lhHostAlloc: bad arguments");
+  *p = nullptr;
+  LH_HIP(hipHostMalloc(p, (size_t)bytes, hipHostMallocDefault));
+  return LH_OK;
+}
+
+int lhHostFree(void *p)
+{
+  if (p)
+    LH_HIP(hipHostFree(p));
+  return LH_OK;
 }
 
 static void drop_graphs(LhContext *c)
@@ -1590,6 +1671,172 @@ int lhBackproject(LhContext *c, uint32_t first, uint32_t count)
     if (rc != LH_OK)
       return rc;
   }
+  return LH_OK;
+}
+
+// One view from host memory into slot `slot`: the H2D copy into the next
+// device ring buffer on upStream (after the staging kernel that last read that
+// buffer), then its staging on stageStream (after the copy).
+static int upload_host_view(LhContext *c, uint32_t slot, int id, const double *matrix,
+    const float *host)
+{
+  const int k        = c->ringNext;
+  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
+  LH_HIP(hipStreamWaitEvent(c->upStream, c->stageDone[k], 0));
+  LH_HIP(hipMemcpyAsync(c->dRing[k], host, bytes, hipMemcpyHostToDevice, c->upStream));
+  LH_HIP(hipEventRecord(c->copyDone[k], c->upStream));
+  LH_HIP(hipStreamWaitEvent(c->stageStream, c->copyDone[k], 0));
+  const int rc = load_view_common(c, slot, id, matrix, c->dRing[k], c->stageStream);
+  if (rc != LH_OK)
+    return rc;
+  LH_HIP(hipEventRecord(c->stageDone[k], c->stageStream));
+  c->ringNext = (k + 1) % c->nRing;
+  return LH_OK;
+}
+
+int lhReconstructHost(LhContext *c, const float *images, const double *matrices, const int *ids,
+    uint32_t count)
+{
+  if (!c || !images || !matrices || !ids || count == 0)
+    return fail(LH_E_ARG, "lhReconstructHost: bad arguments");
+  if (c->ringAcquired >= 0)
+    return fail(LH_E_STATE, "lhReconstructHost: an upload buffer is acquired");
+  const LhConfig &g    = c->cfg;
+  const uint32_t batch = std::min(g.nSlots,
+      g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 16u);
+  const uint32_t nsets = g.nSlots / batch;
+  LH_HIP(hipSetDevice(g.device));
+  while (c->setDone.size() < nsets) {
+    hipEvent_t a = nullptr, b = nullptr;
+    LH_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+    if (hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return fail(LH_E_HIP, "lhReconstructHost: event creation failed");
+    }
+    c->setStaged.push_back(a);
+    c->setDone.push_back(b);
+    c->setUsed.push_back(false);
+  }
+  // staging waits for everything already enqueued on the compute stream
+  // (earlier backprojections may still read the slots)
+  LH_HIP(hipEventRecord(c->setStaged[0], c->stream));
+  LH_HIP(hipStreamWaitEvent(c->stageStream, c->setStaged[0], 0));
+  const size_t px = (size_t)g.W * g.H;
+  uint32_t k = 0;
+  for (uint32_t v0 = 0; v0 < count; v0 += batch, ++k) {
+    const uint32_t set = k % nsets, base = set * batch, n = std::min(batch, count - v0);
+    // the slot set's previous batch must have been backprojected
+    if (c->setUsed[set])
+      LH_HIP(hipStreamWaitEvent(c->stageStream, c->setDone[set], 0));
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t v = v0 + j;
+      const int rc = upload_host_view(c, base + j, ids[v], matrices + 12 * (size_t)v,
+          images + (size_t)v * px);
+      if (rc != LH_OK)
+        return rc;
+    }
+    LH_HIP(hipEventRecord(c->setStaged[set], c->stageStream));
+    LH_HIP(hipStreamWaitEvent(c->stream, c->setStaged[set], 0));
+    const int rc = lhBackproject(c, base, n);
+    if (rc != LH_OK)
+      return rc;
+    LH_HIP(hipEventRecord(c->setDone[set], c->stream));
+    c->setUsed[set] = true;
+  }
+  return LH_OK;
+}
+
+int lhTimedReconstructHost(LhContext *c, const float *images, const double *matrices,
+    const int *ids, uint32_t count, int reps, float *msPerRep)
+{
+  if (!c || reps < 1)
+    return fail(LH_E_ARG, "lhTimedReconstructHost: bad arguments");
+  LH_HIP(hipSetDevice(c->cfg.device));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LH_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(LH_E_HIP, "lhTimedReconstructHost: event creation failed");
+  }
+  double total = 0.0;
+  int rc       = LH_OK;
+  for (int rep = 0; rep < reps && rc == LH_OK; ++rep) {
+    // each repetition starts on idle streams, at its first copy
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(c->stageStream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(c->upStream);
+    if (e == hipSuccess)
+      e = hipEventRecord(e0, c->upStream);
+    if (e != hipSuccess) {
+      rc = fail(LH_E_HIP, "lhTimedReconstructHost: %s", hipGetErrorString(e));
+      break;
+    }
+    rc = lhReconstructHost(c, images, matrices, ids, count);
+    if (rc != LH_OK)
+      break;
+    float ms = 0.0f;
+    e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess)
+      e = hipEventSynchronize(e1);
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+      rc = fail(LH_E_HIP, "lhTimedReconstructHost: %s", hipGetErrorString(e));
+    total += ms;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == LH_OK && msPerRep)
+    *msPerRep = (float)(total / reps);
+  return rc;
+}
+
+int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int reps,
+    float *msPerRep)
+{
+  if (!c || !images || count == 0 || reps < 1)
+    return fail(LH_E_ARG, "lhTimedUploadHost: bad arguments");
+  if (c->ringAcquired >= 0)
+    return fail(LH_E_STATE, "lhTimedUploadHost: an upload buffer is acquired");
+  const size_t px = (size_t)c->cfg.W * c->cfg.H;
+  LH_HIP(hipSetDevice(c->cfg.device));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LH_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(LH_E_HIP, "lhTimedUploadHost: event creation failed");
+  }
+  hipError_t e = hipSuccess;
+  double total = 0.0;
+  for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
+    // the ring buffers are overwritten: every stream that may read them drains first
+    for (hipStream_t st : { c->stream, c->stageStream, c->upStream })
+      if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e == hipSuccess)
+      e = hipEventRecord(e0, c->upStream);
+    for (uint32_t v = 0; v < count && e == hipSuccess; ++v)
+      e = hipMemcpyAsync(c->dRing[v % (uint32_t)c->nRing], images + v * px, px * sizeof(float),
+          hipMemcpyHostToDevice, c->upStream);
+    if (e == hipSuccess)
+      e = hipEventRecord(e1, c->upStream);
+    if (e == hipSuccess)
+      e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&ms, e0, e1);
+    total += ms;
+  }
+  (void)hipStreamSynchronize(c->upStream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess)
+    return fail(LH_E_HIP, "lhTimedUploadHost: %s", hipGetErrorString(e));
+  if (msPerRep)
+    *msPerRep = (float)(total / reps);
   return LH_OK;
 }
 

@@ -296,15 +296,17 @@ static void parallel_copy(float *dst, const float *src, size_t n)
   }
 }
 
-static void flush_batch(void)
+/* Launches the batch being filled; returns 1 if it launched work. */
+static int flush_batch(void)
 {
   if (Pending == 0)
-    return;
+    return 0;
   int rc = lhBackproject(Ctx, BatchBase, Pending);
   if (rc != LH_OK)
     die("backprojection", rc);
   BatchBase = (BatchBase == 0) ? PerPass : 0;
   Pending   = 0;
+  return 1;
 }
 
 int lolaHipBackprojection(RabbitCtGlobalData *rcgd)
@@ -313,7 +315,7 @@ int lolaHipBackprojection(RabbitCtGlobalData *rcgd)
     fprintf(stderr, "LolaHIP: backprojection called before prepare\n");
     exit(EXIT_FAILURE);
   }
-  int last = 0;
+  int launched = 0;
   for (uint32_t p = 0; p < rcgd->numberOfProjections; p++) {
     const Projection *pr = &rcgd->projectionBuffer[p];
     /* the image goes through the context's pinned ring (copied by all host
@@ -326,17 +328,26 @@ int lolaHipBackprojection(RabbitCtGlobalData *rcgd)
     if ((rc = lhCommitUpload(Ctx, BatchBase + Pending, pr->id, pr->matrix)) != LH_OK)
       die("view upload", rc);
     Pending++;
-    if ((uint32_t)pr->id == TotalViews)
-      last = 1;
     if (Pending == PerPass || (uint32_t)pr->id == TotalViews)
-      flush_batch();
+      launched |= flush_batch();
   }
-  if (last) {
+  /* A call that launched a backprojection returns only when that work (and
+   * the uploads enqueued before it) has finished, so the runner's timer
+   * around this call (rabbitRunner.c:55-58,279) measures it.  Views that are
+   * only buffered for the next batch may still be crossing PCIe. */
+  if (launched) {
     int rc = lhSynchronize(Ctx);
     if (rc != LH_OK)
       die("synchronize", rc);
   }
   return 1;
+}
+
+int lolaHipQueryIdle(void)
+{
+  if (Ctx == NULL)
+    return -1;
+  return lhQueryIdle(Ctx);
 }
 
 int lolaHipFinish(RabbitCtGlobalData *rcgd)

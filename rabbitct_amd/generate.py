@@ -18,10 +18,11 @@ def _p(a):
 
 def generate_into(ctx: "lolahip.Context", geom: ScanGeometry, phantom: EllipsoidPhantom,
                   first_slot: int = 0, first_id: int = 1, views=None, border: bool = True,
-                  want_host: bool = False, chunk: int = 16, ids=None):
+                  want_host: bool = False, chunk: int = 16, ids=None, out=None):
     """Generate the views ``views`` (default all) of ``geom`` into slots
     first_slot.. with ids first_id.. (or the 1-based ``ids``, one per view);
-    returns the float32 images if ``want_host``."""
+    returns the float32 images if ``want_host`` (into ``out``, a C-contiguous
+    (n, H, W) float32 array such as a PinnedArray's, when given)."""
     lib = lolahip.load_library()
     views = np.arange(geom.n_views) if views is None else np.asarray(views)
     S, Minv, axis = geom.rays()
@@ -29,7 +30,12 @@ def generate_into(ctx: "lolahip.Context", geom: ScanGeometry, phantom: Ellipsoid
     tau = geom.pixel * geom.sid / geom.sdd
     dbeta = math.radians(geom.arc_deg) / geom.n_views
     ell = np.ascontiguousarray(phantom.params, dtype=np.float64)
-    host = (np.empty((len(views), geom.height, geom.width), np.float32) if want_host else None)
+    shape = (len(views), geom.height, geom.width)
+    if out is not None:
+        if out.shape != shape or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a C-contiguous float32 array of shape {shape}")
+        want_host = True
+    host = (out if out is not None else np.empty(shape, np.float32)) if want_host else None
     ids = (np.arange(first_id, first_id + len(views)) if ids is None else np.asarray(ids))
     ids = np.ascontiguousarray(ids, dtype=np.int32)
     for k0 in range(0, len(views), chunk):
@@ -40,9 +46,9 @@ def generate_into(ctx: "lolahip.Context", geom: ScanGeometry, phantom: Ellipsoid
         mi = np.ascontiguousarray(Minv[sel].reshape(n, 9))
         ax = np.ascontiguousarray(axis[sel])
         pk = np.ascontiguousarray(parker[sel])
-        out = host[k0:k0 + n] if want_host else None
+        dst = host[k0:k0 + n] if want_host else None
         rc = lib.lhGenerateViews(ctx._h, first_slot + k0, n, _p(ids[k0:k0 + n]), _p(m), _p(s),
                                  _p(mi), _p(ax), _p(pk), ell.shape[0], _p(ell), tau, dbeta,
-                                 1 if border else 0, None if out is None else _p(out))
+                                 1 if border else 0, None if dst is None else _p(dst))
         lolahip._check(rc, "lhGenerateViews")
     return host

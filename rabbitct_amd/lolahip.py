@@ -84,6 +84,9 @@ EXPORTS = (
     "lhDeviceAlloc", "lhDeviceFree", "lhCopyToDevice", "lhCopyToHost",
     # before and after the path
     "lhFdkFilter", "lhHuMetric", "lhHuConvert", "lhGetConfig", "lhLoadRctFile",
+    # host-memory reconstruction and the module's timing contract
+    "lhQueryIdle", "lhHostAlloc", "lhHostFree", "lhReconstructHost", "lhTimedReconstructHost",
+    "lolaHipQueryIdle", "lhDeviceInfo", "lhTimedUploadHost",
 )
 
 _lib = None
@@ -170,6 +173,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhHuConvert.argtypes = [P, u64, f32, f32, P]
     lib.lhGetConfig.argtypes = [P, ctypes.POINTER(LhConfig)]
     lib.lhLoadRctFile.argtypes = [P, ctypes.c_char_p, u32, u32, u32, P]
+    lib.lhQueryIdle.argtypes = [P]
+    lib.lhHostAlloc.argtypes = [ctypes.POINTER(P), u64]
+    lib.lhHostFree.argtypes = [P]
+    lib.lhReconstructHost.argtypes = [P, P, P, P, u32]
+    lib.lhTimedReconstructHost.argtypes = [P, P, P, P, u32, i32, ctypes.POINTER(f32)]
+    lib.lolaHipQueryIdle.argtypes = []
+    lib.lhDeviceInfo.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.lhTimedUploadHost.argtypes = [P, P, u32, i32, ctypes.POINTER(f32)]
     _lib = lib
     return lib
 
@@ -229,6 +240,52 @@ def abi_layout():
 
 
 # ---- device layer ---------------------------------------------------------
+
+class PinnedArray:
+    """A numpy array over pinned host memory from lhHostAlloc (H2D copies from
+    it run asynchronously at the DMA rate).  ``array`` is valid until close()."""
+
+    def __init__(self, shape, dtype=np.float32):
+        lib = load_library()
+        self.lib = lib
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        self._p = ctypes.c_void_p()
+        _check(lib.lhHostAlloc(ctypes.byref(self._p), n), "lhHostAlloc")
+        buf = (ctypes.c_char * n).from_address(self._p.value)
+        self.array = np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def close(self):
+        if self._p:
+            self.array = None
+            self.lib.lhHostFree(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_info(device: int = 0) -> dict:
+    """Compute units and peak engine clock (MHz) of a device (lhDeviceInfo)."""
+    cu, khz = ctypes.c_int(), ctypes.c_int()
+    _check(load_library().lhDeviceInfo(device, ctypes.byref(cu), ctypes.byref(khz)),
+           "lhDeviceInfo")
+    return {"compute_units": cu.value, "clock_mhz": khz.value / 1000.0}
+
+
+def module_query_idle() -> int:
+    """lolaHipQueryIdle: 1 when the module has no GPU work in flight, 0 while
+    some runs, -1 before prepare."""
+    return int(load_library().lolaHipQueryIdle())
+
 
 class Context:
     """A reconstruction context: volume, line ranges and projection slots
@@ -353,6 +410,54 @@ class Context:
     def effective_updates(self, count):
         return int(self.lib.lhEffectiveUpdates(self._h, count))
 
+    def query_idle(self) -> bool:
+        rc = self.lib.lhQueryIdle(self._h)
+        if rc < 0:
+            _check(rc, "lhQueryIdle")
+        return rc == 1
+
+    @staticmethod
+    def _host_args(images, matrices, ids):
+        if not (isinstance(images, np.ndarray) and images.dtype == np.float32
+                and images.flags.c_contiguous):
+            raise LolaHipError("images must be a C-contiguous float32 array (pinned memory "
+                               "from PinnedArray for asynchronous copies)")
+        m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1, 12)
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        if images.shape[0] != m.shape[0] or i.shape[0] != m.shape[0]:
+            raise LolaHipError("images, matrices and ids disagree in the number of views")
+        return m, i
+
+    def reconstruct_host(self, images, matrices, ids):
+        """lhReconstructHost: upload, stage and backproject views from host
+        memory in double-buffered batches, then synchronise.  ``images``
+        (N,H,W) float32, C-contiguous (a PinnedArray's array for asynchronous
+        copies)."""
+        m, i = self._host_args(images, matrices, ids)
+        _check(self.lib.lhReconstructHost(self._h, _p(images), _p(m), _p(i), m.shape[0]),
+               "lhReconstructHost")
+        self.synchronize()
+
+    def timed_upload_host(self, images, reps):
+        """lhTimedUploadHost: mean ms per repetition of the H2D copies alone."""
+        if not (isinstance(images, np.ndarray) and images.dtype == np.float32
+                and images.flags.c_contiguous):
+            raise LolaHipError("images must be a C-contiguous float32 array")
+        ms = ctypes.c_float()
+        _check(self.lib.lhTimedUploadHost(self._h, _p(images), images.shape[0], reps,
+                                          ctypes.byref(ms)), "lhTimedUploadHost")
+        return ms.value
+
+    def timed_reconstruct_host(self, images, matrices, ids, reps):
+        """lhTimedReconstructHost: mean ms per repetition, HIP events from
+        before the first H2D copy to after the last backprojection."""
+        m, i = self._host_args(images, matrices, ids)
+        ms = ctypes.c_float()
+        _check(self.lib.lhTimedReconstructHost(self._h, _p(images), _p(m), _p(i), m.shape[0],
+                                               reps, ctypes.byref(ms)),
+               "lhTimedReconstructHost")
+        return ms.value
+
 
 def gather_roof(lds: bool, blocks_per_cu: int = 4, passes: int = 64) -> float:
     """Updates/s of the gather roof microbenchmark (lhGatherRoof)."""
@@ -388,10 +493,12 @@ def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel=
 # ---- runner loop mirror ---------------------------------------------------
 
 def run_module(rct_path, L, buffer_size=1, geometry=True, clip_file=None,
-               algorithm="LolaHIP", volume=None):
+               algorithm="LolaHIP", volume=None, after_iterate=None):
     """Drive a module like rabbitRunner.c:187-287: zeroed L^3 volume, optional
     global geometry (-a), prepare, buffered reads of b views per iterate call,
-    finish.  Returns the float32 (L,L,L) volume."""
+    finish.  ``after_iterate(k)``, when given, runs after the k-th iterate
+    call returns (where the runner's timer stops, rabbitRunner.c:55-58).
+    Returns the float32 (L,L,L) volume."""
     prepare, iterate, finish = algorithm_registry_find(algorithm)
     reader = rct.RctReader(rct_path)
     h = reader.header
@@ -421,10 +528,14 @@ def run_module(rct_path, L, buffer_size=1, geometry=True, clip_file=None,
         buf[i].image = imgs[i].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     data.projectionBuffer = buf
     counter = h.n_images
+    k = 0
     while counter > 0:
         for i in range(buffer_size):
             buf[i].id = reader.read_next(mats[i], imgs[i])
         iterate(ctypes.byref(data))
+        if after_iterate is not None:
+            after_iterate(k)
+        k += 1
         counter -= buffer_size
     finish(ctypes.byref(data))
     return vol

@@ -177,6 +177,49 @@ def test_truncation_extrapolates_left_of_column_zero(closed_geom, kernel, vpp, p
     assert (expect[sel] < 0).any() and (expect[sel] > 0).any()
 
 
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+@pytest.mark.parametrize("kernel,vpp,staging", [("fused", 4, True), ("fused", 4, False),
+                                                ("plain", 1, True)])
+def test_half_storage_alternating_columns(closed_geom, kernel, vpp, prec, staging):
+    """Columns alternating +M and -M (every pixel beside one of opposite sign)
+    make the stored forward difference p(i+1,j) - p(i,j) = -+2M, twice the
+    largest value, the case that bounds the FP16 storage scale (max |p| / s in
+    (4096, 16384], so |d| / s <= 32768 < 65504).  Per view, with u_h the unit
+    roundoff of the storage (FP16 2^-11, BF16 2^-8), p0 and d each carry a
+    relative error <= u_h, so a voxel whose taps lie inside the detector
+    (no extrapolation) is off the FP64 value by at most (3 u_h + 2e-4) M / w^2
+    per view (2e-4 M: FP32 coordinates against a gradient of 2M per pixel).
+    Four views with M from about 0.25 to 4e6 (different FP16 storage scales),
+    each chosen about half a unit in the last place from a storage value, so
+    the rounding is near its worst: every volume value is finite, within that
+    bound, and the error is at least a twentieth of it (the storage really is
+    quantised)."""
+    g = closed_geom
+    W, H = g.width, g.height
+    # FP16: 4097.9 / s lies 1.9 from 4096 (ulp 4) for s = 4^m; BF16: 4111.9
+    # lies 15.9 from 4096 (ulp 32)
+    m0 = 4097.9 if prec == "f16" else 4111.9
+    Ms = m0 * np.array([1.0, 4.0 ** 5, 4.0 ** -7, 4.0 ** 2])
+    sign = np.where(np.arange(W) % 2 == 0, 1.0, -1.0)
+    img = (Ms[:, None, None] * sign[None, None, :] * np.ones((1, H, 1))).astype(np.float32)
+    vol = _bp_closed(g.matrices, img, kernel, vpp, prec, staging)
+    assert np.isfinite(vol).all()
+    ref = orc.backproject(L, voxel_size(L), volume_origin(L), g.matrices, img, mode="bunny")
+    uh = 2.0 ** -11 if prec == "f16" else 2.0 ** -8
+    bound = np.zeros((L, L, L))
+    inside = np.ones((L, L, L), bool)
+    for n in range(g.n_views):
+        u, v, w = _project_grid(g.matrices[n])
+        bound += (3 * uh + 2e-4) * Ms[n] / w ** 2
+        inside &= (u >= 0.01) & (u < W - 1.01) & (v >= 0.01) & (v < H - 1.01)
+    assert inside.sum() > 100000
+    err = np.abs(vol - ref)[inside]
+    ratio = err / bound[inside]
+    print(f"\n{prec} {kernel} staging={staging}: max err / bound {ratio.max():.3f}")
+    assert ratio.max() <= 1.0
+    assert ratio.max() >= 0.05
+
+
 def test_device_line_ranges_byte_equal_reference(golden):
     N, H, W = golden["images"].shape
     for vs in (16, 8):
@@ -207,6 +250,36 @@ def test_module_through_runner_loop(tmp_path, golden):
                 del os.environ["LOLAHIP_VIEWS_PER_PASS"]
             pt, zs = compare_summary(v, g, "opt16")
             assert pt <= TOL and zs <= TOL, (b, vpp, pt)
+
+
+@pytest.mark.parametrize("b,per", [(1, 5), (4, 5), (4, 8), (12, 4), (3, 32)])
+def test_module_calls_that_launch_return_idle(tmp_path, golden_border, b, per):
+    """The runner times each iterate call (rabbitRunner.c:55-58,279): a call
+    that launched a batch must return with no GPU work in flight, so the
+    timer covers it.  Call k holds views [k*b, (k+1)*b); a batch of `per`
+    views launches when the views loaded reach a multiple of per, or view N."""
+    g = golden_border
+    N, H, W = g["images"].shape
+    p = tmp_path / "in.rct"
+    rct.write_rct(p, rct.RctHeader(W, H, N, (1000.0, 0.0)), g["matrices"], g["images"])
+    clip = tmp_path / "clip16.rct"
+    rct.write_clip_file(clip, g["clip16"])
+    idle = []
+    os.environ["LOLAHIP_VIEWS_PER_PASS"] = str(per)
+    try:
+        v = lolahip.run_module(p, L, buffer_size=b, clip_file=clip,
+                               after_iterate=lambda k: idle.append(lolahip.module_query_idle()))
+    finally:
+        del os.environ["LOLAHIP_VIEWS_PER_PASS"]
+    assert len(idle) == N // b
+    per = min(per, N)
+    launching = [k for k in range(N // b)
+                 if ((k + 1) * b) // per > (k * b) // per or (k + 1) * b == N]
+    assert launching
+    assert all(idle[k] == 1 for k in launching), (idle, launching)
+    assert lolahip.module_query_idle() == -1          # after finish: no context
+    pt, zs = compare_summary(v, g, "opt16")
+    assert pt <= TOL and zs <= TOL
 
 
 def test_reference_runner_drives_lolahip(tmp_path, golden_border):
@@ -418,22 +491,24 @@ def test_full_detector_half_staged_vs_fp64(prec):
 
 
 @pytest.mark.parametrize("L2,staging", [(256, True), (256, False), (512, True), (512, False)])
-def test_full_detector_phantom_vs_fp64(L2, staging):
+def test_full_detector_phantom_vs_lolaopt_f32(L2, staging):
     """Full 1248x960 detector, phantom projections: at L=256 the staged kernel
     mixes LDS footprints with the L2 fallback (footprints up to 237 px wide),
     at L=512 every footprint fits.  With 3 views the reference's own FP32
-    arithmetic (LolaOPT.c:131-173, restated in the oracle) is itself about
-    1e-4 of the range away from FP64; the bound is 1e-4 or 1.5x the
-    reference's distance on the same input, whichever is larger."""
+    arithmetic (LolaOPT.c:131-173, restated in the oracle and pinned to the
+    compiled LolaOPT) is itself about 1e-4 of the range away from FP64; the
+    bound is 1e-4 or 1.5x the reference's distance, whichever is larger.  The
+    distance to the FP32 restatement is printed."""
     geom, imgs, ranges, ref = _full_detector_case(L2, 3, 17, noise=False)
     v = lolahip.backproject_views(L2, geom.matrices, imgs, ranges, staging=staging)
     opt = orc.backproject_opt_f32(L2, voxel_size(L2), volume_origin(L2), geom.matrices, imgs,
                                   ranges)
-    rngv = float(ref.max() - ref.min())
-    err = np.abs(v - ref).max() / rngv
-    err_ref = np.abs(opt - ref).max() / rngv
-    print(f"\nL={L2} staging={staging} phantom: max|d|/range {err:.2e} "
-          f"(LolaOPT arithmetic {err_ref:.2e})")
+    rngv = float(opt.max() - opt.min())
+    err_opt = float(np.abs(v - opt).max()) / rngv
+    err = float(np.abs(v - ref).max()) / rngv
+    err_ref = float(np.abs(opt - ref).max()) / rngv
+    print(f"\nL={L2} staging={staging} phantom: max|d|/range vs LolaOPT FP32 {err_opt:.2e}; "
+          f"vs FP64 {err:.2e} (LolaOPT FP32 vs FP64 {err_ref:.2e})")
     assert err <= max(TOL, 1.5 * err_ref)
 
 
@@ -441,13 +516,18 @@ def test_full_detector_phantom_vs_fp64(L2, staging):
 def test_full_detector_noise_vs_fp64(staging):
     """White-noise projections maximise the image gradient, so the FP32
     coordinate rounding that LolaOPT shares (about 1e-4 px at u ~ 1000) shows
-    more: bound max 3e-4 and RMSE 1e-5 of the range."""
+    more: bound max 3e-4 and RMSE 1e-5 of the range against FP64; the
+    distance to LolaOPT's FP32 arithmetic is printed beside it."""
     geom, imgs, ranges, ref = _full_detector_case(256, 3, 19, noise=True)
     v = lolahip.backproject_views(256, geom.matrices, imgs, ranges, staging=staging)
+    opt = orc.backproject_opt_f32(256, voxel_size(256), volume_origin(256), geom.matrices, imgs,
+                                  ranges)
     rngv = float(ref.max() - ref.min())
     d = v - ref
     mx, rmse = np.abs(d).max() / rngv, np.sqrt((d ** 2).mean()) / rngv
-    print(f"\nnoise staging={staging}: max {mx:.2e} rmse {rmse:.2e}")
+    print(f"\nnoise staging={staging}: max {mx:.2e} rmse {rmse:.2e}; vs LolaOPT FP32 max "
+          f"{np.abs(v - opt).max() / rngv:.2e}; LolaOPT FP32 vs FP64 max "
+          f"{np.abs(opt - ref).max() / rngv:.2e}")
     assert mx <= 3e-4 and rmse <= 1e-5
 
 

@@ -1,0 +1,125 @@
+"""Parity at the launch shapes the benchmark times, through the C-ABI.
+
+The L=512 workload (full 1248 x 960 detector, the seed-512 scan and phantom
+the benchmark reconstructs) on 62 of its 496 views (every 8th), with the
+fused kernel at 64 and 8 views per register pass, 16- and 32-voxel runs,
+LDS-staged footprints, in FP32, BF16 and FP16 storage.  64 sampled lines
+(boundary and centre lines included) are compared with the FP64 oracle
+(LolaOPT.c:131-173 semantics in double precision):
+  FP32: max |d| <= 1e-4 of the sampled range (the stated bar);
+  BF16 / FP16: RMSE <= 1e-3 of the range (the stated bar), and a regression
+  bound a few times the measured error.
+The host-memory pipeline (lhReconstructHost: H2D copies, staging on its own
+stream, double-buffered slot sets, graph-captured batches) must equal the
+HBM-resident path bit for bit at the same launch shape."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from rabbitct_amd import lolahip
+from rabbitct_amd.generate import generate_into
+from rabbitct_amd.phantom import EllipsoidPhantom, ScanGeometry, voxel_size, volume_origin
+
+pytestmark = pytest.mark.gpu
+
+L = 512
+SEED = 512
+STEP = 8
+TOL = 1e-4
+# regression bounds on RMSE / range against FP64, a few times the error the
+# half-precision storage shows on this input
+HALF_RMSE = {"bf16": 2e-4, "f16": 3e-5}
+
+
+@pytest.fixture(scope="module")
+def headline():
+    geom = ScanGeometry(n_views=496, seed=SEED).build()
+    sel = np.arange(0, 496, STEP)
+    ph = EllipsoidPhantom(seed=SEED).build()
+    with lolahip.Context(L, geom.width, geom.height, 496, n_slots=len(sel)) as ctx:
+        imgs = generate_into(ctx, geom, ph, views=sel, want_host=True, ids=sel + 1)
+    mats = geom.matrices[sel]
+    ranges = lolahip.compute_line_ranges(L, geom.width, geom.height, mats, 16, device=True)
+    rng = np.random.default_rng(512)
+    zs = np.concatenate([[0, L - 1, L // 2, L // 2, 8, L - 9], rng.integers(0, L, 58)])
+    ys = np.concatenate([[L // 2, L // 2, 0, L - 1, L // 2, L // 3], rng.integers(0, L, 58)])
+    ref = orc.backproject_lines(L, voxel_size(L), volume_origin(L), mats, imgs, ranges, zs, ys)
+    return {"mats": mats, "imgs": imgs, "ranges": ranges, "zs": zs, "ys": ys, "ref": ref}
+
+
+def _lines(vol, h):
+    return vol[h["zs"], h["ys"], :].astype(np.float64)
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("vpp,run", [(64, 16), (64, 32), (8, 16), (8, 32)])
+def test_headline_shape_vs_fp64_lines(headline, vpp, run, prec):
+    h = headline
+    print(f"\ncase: L=512 62 views fused vpp={vpp} R={run} {prec} staged", flush=True)
+    vol = lolahip.backproject_views(L, h["mats"], h["imgs"], h["ranges"], precision=prec,
+                                    kernel="fused", views_per_pass=vpp, run_length=run,
+                                    staging=True)
+    got = _lines(vol, h)
+    del vol
+    ref = h["ref"]
+    span = float(ref.max() - ref.min())
+    d = np.abs(got - ref)
+    mx, rmse = float(d.max()) / span, float(np.sqrt((d ** 2).mean())) / span
+    print(f"vs FP64: max {mx:.2e} rmse {rmse:.2e} of range")
+    if prec == "f32":
+        assert mx <= TOL
+    else:
+        assert rmse <= 1e-3
+        assert rmse <= HALF_RMSE[prec]
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("prec,vpp", [("f32", 16), ("f32", 64), ("f16", 8), ("bf16", 16)])
+def test_host_pipeline_equals_resident_path(headline, prec, vpp, graphs):
+    """lhReconstructHost from pinned host memory (batches of vpp views over
+    two slot sets: the upload and staging of batch k+1 overlap the
+    backprojection of batch k) gives the volume of the resident path bit for
+    bit: the same staging kernel fills the slots and the launches cover the
+    same view groups in the same order.  Run twice in a row into the same
+    volume: the second run equals the resident path started from the first
+    run's volume."""
+    h = headline
+    n = len(h["mats"])
+    print(f"\ncase: host pipeline {prec} vpp={vpp} graphs={graphs}", flush=True)
+    ref = lolahip.backproject_views(L, h["mats"], h["imgs"], h["ranges"], precision=prec,
+                                    kernel="fused", views_per_pass=vpp, run_length=16)
+    H, W = h["imgs"].shape[1:]
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    with lolahip.PinnedArray(h["imgs"].shape) as pin:
+        pin.array[:] = h["imgs"]
+        with lolahip.Context(L, W, H, n, n_slots=2 * vpp, precision=prec, kernel="fused",
+                             views_per_pass=vpp, run_length=16) as ctx:
+            ctx.set_ranges(h["ranges"])
+            ctx.set_graphs(graphs)
+            ctx.reconstruct_host(pin.array, h["mats"], ids)
+            assert ctx.query_idle()
+            once = ctx.get_volume()
+            ctx.reconstruct_host(pin.array, h["mats"], ids)
+            twice = ctx.get_volume()
+    assert np.array_equal(once.view(np.uint32), ref.view(np.uint32))
+    # the second run adds the passes onto the first run's volume, as the
+    # resident path does when it starts from that volume
+    ref2 = lolahip.backproject_views(L, h["mats"], h["imgs"], h["ranges"], precision=prec,
+                                     kernel="fused", views_per_pass=vpp, run_length=16,
+                                     volume=ref)
+    assert np.array_equal(twice.view(np.uint32), ref2.view(np.uint32))
+
+
+def test_timed_host_pipeline_reports_a_time(headline):
+    h = headline
+    n = len(h["mats"])
+    H, W = h["imgs"].shape[1:]
+    with lolahip.PinnedArray(h["imgs"].shape) as pin:
+        pin.array[:] = h["imgs"]
+        with lolahip.Context(L, W, H, n, n_slots=32, kernel="fused", views_per_pass=16) as ctx:
+            ctx.set_ranges(h["ranges"])
+            ms = ctx.timed_reconstruct_host(pin.array, h["mats"],
+                                            np.arange(1, n + 1, dtype=np.int32), 2)
+            assert ctx.query_idle()
+    print(f"\nhost pipeline, 62 views at L=512: {ms:.2f} ms per run")
+    assert ms > 0.0

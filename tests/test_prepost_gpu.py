@@ -60,3 +60,49 @@ def test_fdk_filter_matches_generator_and_oracle():
     # view count, the filter the scan's
     ref *= len(views) / geom.n_views
     assert np.abs(got - ref).max() <= 2e-5 * scale
+
+
+def test_fdk_cosine_weight_and_ramp_taps_closed_form():
+    """Closed form of the FDK pre-processing on unit impulses, on an untilted,
+    unjittered geometry with an odd detector (principal point at pixel
+    (52, 40)): one impulse per row at (i, j) leaves, after the cosine weight
+    and the Ram-Lak row filter (Kak & Slaney: h(0) = 1/(4 tau^2),
+    h(k odd) = -1/(pi^2 k^2 tau^2), h(k even) = 0, times tau and d_beta),
+      at (i, j):    d_beta * cos / (4 tau)
+      at (i+-1, j): -d_beta * cos / (pi^2 tau)
+      at (i+-2, j): 0
+    with cos = SDD / sqrt(SDD^2 + r^2), r the distance of the pixel from the
+    principal point on the detector (mm); cos = 1 on the principal ray."""
+    lib = lolahip.load_library()
+    W, H, nv = 105, 81, 3
+    g = ScanGeometry(n_views=nv, width=W, height=H, pixel=0.3, jitter_mm=0.0, jitter_deg=0.0,
+                     seed=5).build()
+    pts = [(52, 40), (10, 5), (100, 70), (52, 20), (90, 60), (3, 77), (60, 41)]
+    raw = np.zeros((nv, H, W), np.float32)
+    for i, j in pts:
+        raw[:, j, i] = 1.0
+    _, Minv, axis = g.rays()
+    mi = np.ascontiguousarray(Minv.reshape(nv, 9))
+    ax = np.ascontiguousarray(axis)
+    parker = np.ones((nv, W), np.float32)
+    tau = g.pixel * g.sid / g.sdd
+    dbeta = 0.0125
+    with unet.DeviceBuffer.from_array(raw) as dr, unet.DeviceBuffer(raw.nbytes) as do:
+        rc = lib.lhFdkFilter(W, H, nv, mi.ctypes.data, ax.ctypes.data, parker.ctypes.data, tau,
+                             dbeta, 0, unet._addr(dr), unet._addr(do))
+        assert rc == 0, lib.lhLastError()
+        out = do.download((nv, H, W), np.float32).astype(np.float64)
+    cu, cv = (W - 1) / 2, (H - 1) / 2
+    for i, j in pts:
+        r2 = ((i - cu) ** 2 + (j - cv) ** 2) * g.pixel ** 2
+        cosw = g.sdd / np.sqrt(g.sdd ** 2 + r2)
+        np.testing.assert_allclose(out[:, j, i], dbeta * cosw / (4 * tau), rtol=2e-6)
+        for k in (-1, 1):
+            if 0 <= i + k < W:
+                np.testing.assert_allclose(out[:, j, i + k], -dbeta * cosw / (np.pi ** 2 * tau),
+                                           rtol=2e-6)
+        for k in (-2, 2):
+            if 0 <= i + k < W:
+                assert np.all(out[:, j, i + k] == 0.0)
+    # the principal ray: weight 1
+    np.testing.assert_allclose(out[:, 40, 52] * 4 * tau / dbeta, 1.0, rtol=1e-6)
