@@ -160,140 +160,92 @@ __device__ __forceinline__ float lerp_pair(uint32_t q, float ax)
     return fmaf(ax, bf16_hi(q), bf16_lo(q));
 }
 
-// The four taps of the bilinear stencil at integer corner (iix, iiy), both
-// already clamped to [-2, W] x [-2, H].
-//  fetch_taps (FP32): row-major padded image, four 4-byte loads (rows iiy
-//  and iiy+1).
-//  fetch_pairs (BF16/F16): "pair" layout, element (i,j) holds p(i,j) and
-//  p(i+1,j) - p(i,j) as 2 halves in one dword: two 4-byte loads (rows iiy and
-//  iiy+1), returned packed.
-template <class Args>
-__device__ __forceinline__ void fetch_taps(__amdgpu_buffer_rsrc_t rs, Args &A,
-    int iix, int iiy, float &p00, float &p10, float &p01, float &p11)
+// Coordinates.  Each tile-view has a footprint box (box_kernel) whose origin
+// (iu0, iv0) is an integer pixel; voxel coordinates are carried relative to
+// it, u' = u - iu0 w and v' = v - iv0 w, from an FP64 origin per run (the
+// subtraction is done in FP64, so the integer offset costs nothing in
+// accuracy).  The local detector coordinates ix' = u'/w, iy' = v'/w are at
+// most the box size (< 128 px where the footprint is staged) instead of up to
+// the detector size, so their FP32 rounding is 8-16x smaller than that of
+// global coordinates near pixel 1000.  Three wave-uniform modes per tile-view:
+//  kInterior  the box lies inside the detector and every coordinate is >= 1
+//             px: no clamp; the local coordinates are >= 1, so truncation is
+//             the floor and x - trunc(x) is fract(x) exactly;
+//  kClamp     coordinates clamped to the box [0, fw-2] x [0, fh-2]; the box
+//             origin is >= 0, so the global coordinate is >= 0 and its
+//             truncation is the local floor;
+//  kEdge      clamped, and the box origin is left of or above pixel 0: the
+//             global coordinate ix' + iu0 (exact: |iu0| <= 2) is truncated
+//             toward zero, as LolaOPT.c:144-173 truncates (negative fractions
+//             extrapolate in (-1, 0)).
+enum { kInterior = 0, kClamp = 1, kEdge = 2 };
+
+// Local cell of a coordinate pair: fractions (ax, ay) and the integer corner
+// (cx, cy), relative to the box origin (kInterior, kClamp) or global (kEdge).
+template <int MODE>
+__device__ __forceinline__ void cell_of(float ix, float iy, float hiu, float hiv, float ou,
+    float ov, float &ax, float &ay, float &fx, float &fy)
 {
-  const int pitch    = (int)(A.rowBytes >> 2);
-  const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
-  p00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-  p10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0));
-  p01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0));
-  p11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, (int)A.rowBytes, 0));
+  if constexpr (MODE == kInterior) {
+    ax = __builtin_amdgcn_fractf(ix);
+    ay = __builtin_amdgcn_fractf(iy);
+    fx = ix;   // converted by truncation by the caller
+    fy = iy;
+  } else {
+    ix = __builtin_amdgcn_fmed3f(ix, 0.0f, hiu);
+    iy = __builtin_amdgcn_fmed3f(iy, 0.0f, hiv);
+    if constexpr (MODE == kEdge) {
+      ix += ou;
+      iy += ov;
+    }
+    fx = __builtin_truncf(ix);
+    fy = __builtin_truncf(iy);
+    ax = ix - fx;
+    ay = iy - fy;
+  }
 }
 
-template <class Args>
-__device__ __forceinline__ void fetch_pairs(__amdgpu_buffer_rsrc_t rs, Args &A,
-    int iix, int iiy, uint32_t &q0, uint32_t &q1)
+// The four taps of the bilinear stencil from the slot image in L2, at byte
+// offset `off` of the upper-left tap.
+//  FP32: row-major bordered image, four 4-byte loads (rows j and j+1).
+//  BF16/F16: "pair" layout, element (i,j) holds p(i,j) and p(i+1,j) - p(i,j)
+//  as 2 halves in one dword: two 4-byte loads (rows j and j+1).
+template <int PREC>
+__device__ __forceinline__ float l2_lerp(__amdgpu_buffer_rsrc_t rs, uint32_t off, uint32_t row,
+    float ax, float ay)
 {
-  const int pitch    = (int)(A.rowBytes >> 2);
-  const uint32_t off = (uint32_t)(__mul24(iiy, pitch) + iix) * 4u + A.baseOffset;
-  q0 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
-  q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)A.rowBytes, 0);
-}
-
-// One voxel-view update: returns val / w^2 for the voxel at homogeneous
-// (u, v, w) (LolaOPT.c:145-170).
-template <int PREC, class Args>
-__device__ __forceinline__ float voxel_update(__amdgpu_buffer_rsrc_t rs, Args &A,
-    float u, float v, float w)
-{
-  const float rw = __builtin_amdgcn_rcpf(w);
-  const float ix = __builtin_amdgcn_fmed3f(u * rw, -2.0f, A.Wf);
-  const float iy = __builtin_amdgcn_fmed3f(v * rw, -2.0f, A.Hf);
-  const float fx = __builtin_truncf(ix);
-  const float fy = __builtin_truncf(iy);
-  const float ax = ix - fx;
-  const float ay = iy - fy;
   float b, t;
   if constexpr (PREC == LH_PREC_F32) {
-    float p00, p10, p01, p11;
-    fetch_taps(rs, A, (int)fx, (int)fy, p00, p10, p01, p11);
+    const float p00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+    const float p10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, 0, 0));
+    const float p01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)row, 0));
+    const float p11 =
+        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4u, (int)row, 0));
     b = fmaf(ax, p10 - p00, p00);
     t = fmaf(ax, p11 - p01, p01);
   } else {
-    uint32_t q0, q1;
-    fetch_pairs(rs, A, (int)fx, (int)fy, q0, q1);
+    const uint32_t q0 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+    const uint32_t q1 = __builtin_amdgcn_raw_buffer_load_b32(rs, off, (int)row, 0);
     b = lerp_pair<PREC>(q0, ax);
     t = lerp_pair<PREC>(q1, ax);
   }
-  const float val = fmaf(ay, t - b, b);
-  return val * (rw * rw);
+  return fmaf(ay, t - b, b);
 }
 
-template <int R, int PREC>
-__device__ __forceinline__ void bp_tile(const BpArgs &A, int tx, int ty, int tz)
+// One voxel-view update with taps from L2 (LolaOPT.c:145-170): returns
+// val / w^2 for the voxel at local homogeneous (u', v', w).  `base` is the
+// byte offset of the box origin pixel in the slot (kEdge: of pixel (0, 0)),
+// `pitch` the row pitch in elements.  The coordinate arithmetic is that of
+// lds_update, so both tap paths give the same bits.
+template <int PREC, int MODE>
+__device__ __forceinline__ float l2_update(__amdgpu_buffer_rsrc_t rs, uint32_t base, int pitch,
+    uint32_t row, float u, float v, float w, float hiu, float hiv, float ou, float ov)
 {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv   = threadIdx.x / kWave;
-  const int x0   = tx * R;
-  const int y    = ty * kWave + lane;
-  const int z    = tz * kWaves + wv;
-  const int L    = A.L;
-
-  const double wx0 = (double)A.O_L + (double)x0 * (double)A.R_L;
-  const double wy  = (double)A.O_L + (double)y * (double)A.R_L;
-  const double wz  = (double)A.O_L + (double)z * (double)A.R_L;
-  const uint64_t line = (uint64_t)z * (uint64_t)L + (uint64_t)y;
-  const uint64_t LL   = (uint64_t)L * (uint64_t)L;
-
-  float acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    acc[r] = 0.0f;
-  bool touched = false;
-
-  // line range of the first view, then prefetch one view ahead
-  int rr_next = A.clip ? A.ranges[(uint64_t)(view_params(A, A.firstSlot).id - 1) * LL + line]
-                       : (L << 16);
-  for (int v = 0; v < A.nViews; ++v) {
-    const int slot       = A.firstSlot + v;
-    CViewParams &P       = view_params(A, slot);
-    const int rr         = rr_next;
-    if (A.clip && v + 1 < A.nViews)
-      rr_next = A.ranges[(uint64_t)(view_params(A, slot + 1).id - 1) * LL + line];
-    const int start = (int)(short)(rr & 0xffff);
-    const int stop  = rr >> 16;
-    const int rlo   = max(start - x0, 0);
-    const int rhi   = min(stop - x0, R);
-    const bool active = rlo < rhi;
-    if (!__any(active))
-      continue;
-    touched |= active;
-
-    // FP64 per-line origin, as in the staged kernel
-    const float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
-    const float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
-    const float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
-    const float du = P.du, dv = P.dv, dw = P.dw;
-    const __amdgpu_buffer_rsrc_t rs = slot_rsrc(A, slot);
-
-    if (__all(rlo == 0 && rhi == R)) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const float fr = (float)r;
-        acc[r] += voxel_update<PREC>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0), fmaf(fr, dw, w0));
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (r >= rlo && r < rhi) {
-          const float fr = (float)r;
-          acc[r] += voxel_update<PREC>(rs, A, fmaf(fr, du, u0), fmaf(fr, dv, v0), fmaf(fr, dw, w0));
-        }
-      }
-    }
-  }
-
-  if (touched) {
-    float4 *p = reinterpret_cast<float4 *>(A.vol + line * (uint64_t)L + (uint64_t)x0);
-#pragma unroll
-    for (int q = 0; q < R / 4; ++q) {
-      float4 o = p[q];
-      o.x += acc[4 * q + 0];
-      o.y += acc[4 * q + 1];
-      o.z += acc[4 * q + 2];
-      o.w += acc[4 * q + 3];
-      p[q] = o;
-    }
-  }
+  const float rw = __builtin_amdgcn_rcpf(w);
+  float ax, ay, fx, fy;
+  cell_of<MODE>(u * rw, v * rw, hiu, hiv, ou, ov, ax, ay, fx, fy);
+  const uint32_t off = (uint32_t)(__mul24((int)fy, pitch) + (int)fx) * 4u + base;
+  return l2_lerp<PREC>(rs, off, row, ax, ay) * (rw * rw);
 }
 
 // ---- LDS-staged tile ------------------------------------------------------
@@ -365,6 +317,9 @@ __device__ __forceinline__ uint2 fetch_box(__amdgpu_buffer_rsrc_t rs, Args &A, i
   return make_uint2(q[0], q[1]);
 }
 
+// STAGE = 0: every footprint is gathered from L2 (the staged tile shape and
+// coordinate arithmetic without the LDS copies).
+template <int STAGE>
 __device__ __forceinline__ StageBox decode_box(uint2 raw)
 {
   const int qx = __builtin_amdgcn_readfirstlane((int)raw.x);
@@ -375,7 +330,7 @@ __device__ __forceinline__ StageBox decode_box(uint2 raw)
   b.fw       = qy & 0x7fff;
   b.interior = (qy & 0x8000) != 0;
   b.fh       = qy >> 16;
-  b.fits     = b.fw >= 2 && b.fh >= 2 && b.fw <= kFW && b.fh <= kFH;
+  b.fits     = STAGE && b.fw >= 2 && b.fh >= 2 && b.fw <= kFW && b.fh <= kFH;
   return b;
 }
 
@@ -466,38 +421,30 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
   }
 }
 
-// One voxel-view update from the LDS footprint (LolaOPT.c:145-170): sBase[v *
-// kPitch + u] is footprint pixel (u, v).  CLAMP: coordinates clamped to the
-// box [lou, hiu] x [lov, hiv], truncation toward zero.  Without CLAMP (interior
-// boxes) the coordinates are positive and inside the box, so trunc is the
-// floor, x - trunc(x) is fract(x) exactly and the conversion truncates.
-template <int PREC, bool CLAMP>
+// One voxel-view update from the LDS footprint (LolaOPT.c:145-170) at local
+// homogeneous (u', v', w): returns val / w^2.  sBase[v * kPitch + u] is
+// footprint pixel (u, v) (kEdge: sBase is offset by the box origin, so it is
+// indexed by global pixels).  The coordinate arithmetic is cell_of's, shared
+// with l2_update.
+template <int PREC, int MODE>
 __device__ __forceinline__ float lds_update(const float *sBase, float u, float v, float w,
-    float lou, float hiu, float lov, float hiv)
+    float hiu, float hiv, float ou, float ov)
 {
   const float rw = __builtin_amdgcn_rcpf(w);
-  float ix = u * rw, iy = v * rw;
-  float ax, ay;
+  float ax, ay, fx, fy;
+  cell_of<MODE>(u * rw, v * rw, hiu, hiv, ou, ov, ax, ay, fx, fy);
   const float *q;
-  if constexpr (CLAMP) {
-    ix = __builtin_amdgcn_fmed3f(ix, lou, hiu);
-    iy = __builtin_amdgcn_fmed3f(iy, lov, hiv);
-    const float fx = __builtin_truncf(ix);
-    const float fy = __builtin_truncf(iy);
-    // fy*kPitch + fx is an exact integer in FP32 (|.| < 2^24): one conversion
-    q  = sBase + (int)fmaf(fy, (float)kPitch, fx);
-    ax = ix - fx;
-    ay = iy - fy;
-  } else {
-    ax = __builtin_amdgcn_fractf(ix);
-    ay = __builtin_amdgcn_fractf(iy);
+  if constexpr (MODE == kInterior) {
     // byte address base + 4 * column + row * 4 * kPitch as two shift-adds: the
     // empty asm hides the partial sum, which the compiler would otherwise
     // re-associate into two shifts and a three-way add
     typedef const __attribute__((address_space(3))) float LdsFloat;
-    uint32_t col = (uint32_t)(uintptr_t)(LdsFloat *)sBase + ((uint32_t)(int)ix << 2);
+    uint32_t col = (uint32_t)(uintptr_t)(LdsFloat *)sBase + ((uint32_t)(int)fx << 2);
     __asm__("" : "+v"(col));
-    q = (const float *)(LdsFloat *)(uintptr_t)(col + ((uint32_t)(int)iy << (kPitchLog2 + 2)));
+    q = (const float *)(LdsFloat *)(uintptr_t)(col + ((uint32_t)(int)fy << (kPitchLog2 + 2)));
+  } else {
+    // fy*kPitch + fx is an exact integer in FP32 (|.| < 2^24): one conversion
+    q = sBase + (int)fmaf(fy, (float)kPitch, fx);
   }
   float b, t;
   if constexpr (PREC == LH_PREC_F32) {
@@ -514,12 +461,13 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
   return fmaf(ay, t - b, b) * (rw * rw);
 }
 
-template <int R, int PREC>
+template <int R, int PREC, int STAGE>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
 {
   constexpr int ELEM    = 4;         // FP32 pixel or half pair: one dword per element
-  constexpr float kPark = 0x1p100f;  // w of a parked lane: (1/w)^2 underflows to +0
+  constexpr float kPark   = 0x1p100f; // w of a parked lane: (1/w)^2 underflows to +0
+  constexpr float kParkUV = 0x1p99f;  // its u' and v': coordinate (1/2, 1/2) in the box
   const int tile = tx + A.tilesX * (ty + A.tilesY * tz);
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -567,7 +515,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   __syncthreads();
   bool blockAct = __builtin_amdgcn_readfirstlane(sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
   const __amdgpu_buffer_rsrc_t boxRs = box_rsrc(A);
-  StageBox box = decode_box(fetch_box(boxRs, A, A.firstSlot, tile));
+  StageBox box = decode_box<STAGE>(fetch_box(boxRs, A, A.firstSlot, tile));
   // box of view v+1, in flight during view v-1
   uint2 boxRaw = n > 1 ? fetch_box(boxRs, A, A.firstSlot + 1, tile) : make_uint2(0u, 0u);
   if (blockAct && box.fits)
@@ -591,7 +539,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       rr_nxt = range_of(v + 2);
     StageBox boxNext = box;
     if (hasNext) {
-      boxNext = decode_box(boxRaw);
+      boxNext = decode_box<STAGE>(boxRaw);
       KernelArgs &K = kernel_args();
       if (v + 2 < n)
         boxRaw = fetch_box(box_rsrc(K), K, slot + 2, tile);
@@ -607,81 +555,116 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     const bool active = rlo < rhi;
     if (blockAct && __any(active)) {
       touched |= active;
-      // FP64 per-line origin A (x0, y, z, 1), rounded once to FP32; the run's
-      // voxels step from it in FP32, one rounding each, so nothing drifts
-      // along x at any L
-      float u0 = (float)fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
-      float v0 = (float)fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
-      float w0 = (float)fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
-      const float du = P.du, dv = P.dv, dw = P.dw;
-      // Voxels outside their line's range are "parked": (u, v, w) = (pu, pv, 1)
-      // * 2^100 puts the coordinate at (pu, pv), inside the footprint half a
-      // pixel from its corner, and the weight (1/w)^2 underflows to +0, so the
-      // update adds exactly zero.  The accumulators then take one straight
-      // path through every branch below.
-      // (iu0 + 0.5) * 2^100 = (2 iu0 + 1) * 2^99, exact
-      const float pu = box.fits ? ldexpf((float)(2 * box.iu0 + 1), -1) : 0.0f;
-      const float pv = box.fits ? ldexpf((float)(2 * box.iv0 + 1), -1) : 0.0f;
+      // FP64 per-line origin A (x0, y, z, 1) relative to the box origin:
+      // u' = u - iu0 w, v' = v - iv0 w in FP64, then rounded once to FP32; the
+      // run's voxels step from it in FP32, one rounding each, so nothing
+      // drifts along x at any L
+      const double U0 = fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
+      const double V0 = fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
+      const double W0 = fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
+      float u0 = (float)fma(-(double)box.iu0, W0, U0);
+      float v0 = (float)fma(-(double)box.iv0, W0, V0);
+      float w0 = (float)W0;
+      // the x-steps of u' and v': a*R_L - iu0 * a[2]*R_L
+      const float dw = P.dw;
+      const float du = fmaf(-(float)box.iu0, dw, P.du), dv = fmaf(-(float)box.iv0, dw, P.dv);
+      // Voxels outside their line's range are "parked": (u', v', w) =
+      // (1/2, 1/2, 1) * 2^100 puts the coordinate half a pixel inside the box
+      // and the weight (1/w)^2 underflows to +0, so the update adds exactly
+      // zero.  The accumulators then take one straight path through every
+      // branch below.
       const bool full = __all(!active || (rlo == 0 && rhi == R));
       if (full && !__all(active) && !active) {
-        u0 = box.fits ? ldexpf((float)(2 * box.iu0 + 1), 99) : 0.0f;
-        v0 = box.fits ? ldexpf((float)(2 * box.iv0 + 1), 99) : 0.0f;
+        u0 = kParkUV;
+        v0 = kParkUV;
         w0 = kPark;
       }
+      // Each update loop starts from its own opaque copy of the run origin
+      // (an empty asm naming the loop): the loops then share no leading
+      // instructions, which the compiler would otherwise hoist (with their
+      // reciprocals, clamps and bound conversions) into the block every path
+      // runs.  The clamp bounds are converted inside the loops that use them.
+#define LH_LOOP_ORIGIN(tag)                                                              \
+  float uo = u0, vo = v0, wo = w0;                                                       \
+  __asm__("; " tag : "+v"(uo), "+v"(vo), "+v"(wo));                                      \
+  const float hiu = (float)(box.fw - 2), hiv = (float)(box.fh - 2)
+      // Boxes whose origin is left of or above pixel 0 (the detector edge)
+      // truncate the global coordinate ix' + iu0; the others add 0, which
+      // leaves the local coordinate as it is (kEdge arithmetic with a zero
+      // offset is kClamp's).
+      const bool edge = box.iu0 < 0 || box.iv0 < 0;
       if (box.fits) {
-        // sBase[v*kPitch + u] is footprint pixel (u, v): the window origin folded in
-        const float *sBase = sImg + (v & 1) * kPitch * kFH - (box.iv0 * kPitch + box.iu0);
-        const float lou = (float)box.iu0, hiu = (float)(box.iu0 + box.fw - 2);
-        const float lov = (float)box.iv0, hiv = (float)(box.iv0 + box.fh - 2);
+        // sImg buffer of this view
+        const float *sBuf = sImg + (v & 1) * kPitch * kFH;
         // groups of kFence updates are fenced so the scheduler keeps at most
         // that many updates' taps in flight per lane
         if (full && box.interior) {
+          LH_LOOP_ORIGIN("lds interior");
           // with VECTORSIZE-aligned ranges a run is all in or all out
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
-            acc[r] += lds_update<PREC, false>(sBase, r ? fmaf(fr, du, u0) : u0,
-                r ? fmaf(fr, dv, v0) : v0, r ? fmaf(fr, dw, w0) : w0, lou, hiu, lov, hiv);
+            acc[r] += lds_update<PREC, kInterior>(sBuf, r ? fmaf(fr, du, uo) : uo,
+                r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
             if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
-        } else if (full) {
+        } else if (full && !edge) {
+          LH_LOOP_ORIGIN("lds clamp");
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
-            acc[r] += lds_update<PREC, true>(sBase, r ? fmaf(fr, du, u0) : u0,
-                r ? fmaf(fr, dv, v0) : v0, r ? fmaf(fr, dw, w0) : w0, lou, hiu, lov, hiv);
+            acc[r] += lds_update<PREC, kClamp>(sBuf, r ? fmaf(fr, du, uo) : uo,
+                r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
             if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         } else {
+          LH_LOOP_ORIGIN("lds edge or partial");
+          // detector-edge boxes and partial runs, with
+          // per-voxel parking (parked coordinates lie inside every box); for
+          // an interior box the clamp is the identity and the bits are those
+          // of the interior loop (for coordinates >= 0, trunc is the floor
+          // and x - trunc(x) is fract(x))
+          const float ou = edge ? (float)box.iu0 : 0.0f, ov = edge ? (float)box.iv0 : 0.0f;
+          // indexed by global pixels for edge boxes
+          const float *sB = edge ? sBuf - (box.iv0 * kPitch + box.iu0) : sBuf;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
             const bool on  = r >= rlo && r < rhi;
-            acc[r] += lds_update<PREC, true>(sBase, on ? fmaf(fr, du, u0) : pu * kPark,
-                on ? fmaf(fr, dv, v0) : pv * kPark, on ? fmaf(fr, dw, w0) : kPark, lou, hiu,
-                lov, hiv);
+            acc[r] += lds_update<PREC, kEdge>(sB, on ? fmaf(fr, du, uo) : kParkUV,
+                on ? fmaf(fr, dv, vo) : kParkUV, on ? fmaf(fr, dw, wo) : kPark, hiu, hiv, ou,
+                ov);
             if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
           }
         }
       } else {
-        // footprint larger than the LDS window: L2 gathers, one voxel at a time
+        // footprint larger than the LDS window (or taps from L2 by
+        // configuration): L2 gathers in groups of kFenceL2 voxels with the
+        // clamp loop's arithmetic
+        LH_LOOP_ORIGIN("l2");
         KernelArgs &K = kernel_args();
         const __amdgpu_buffer_rsrc_t rs = slot_rsrc(K, slot);
+        const int pitchE   = (int)(K.rowBytes >> 2);
+        const uint32_t row = K.rowBytes;
+        const float ou = edge ? (float)box.iu0 : 0.0f, ov = edge ? (float)box.iv0 : 0.0f;
+        // byte offset of the box origin pixel (edge boxes: of pixel (0, 0))
+        const uint32_t base =
+            K.baseOffset + (edge ? 0u : (uint32_t)(box.iv0 * pitchE + box.iu0) * 4u);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const float fr = (float)r;
           const bool on  = r >= rlo && r < rhi;
-          acc[r] += voxel_update<PREC>(rs, K, on ? fmaf(fr, du, u0) : 0.0f,
-              on ? fmaf(fr, dv, v0) : 0.0f, on ? fmaf(fr, dw, w0) : kPark);
-          // groups of kFenceL2 updates keep that many voxels' tap loads in
-          // flight per lane
+          acc[r] += l2_update<PREC, kEdge>(rs, base, pitchE, row,
+              on ? fmaf(fr, du, uo) : kParkUV, on ? fmaf(fr, dv, vo) : kParkUV,
+              on ? fmaf(fr, dw, wo) : kPark, hiu, hiv, ou, ov);
           if ((r % kFenceL2<R>) == kFenceL2<R> - 1)
             __builtin_amdgcn_sched_barrier(0);
         }
       }
+#undef LH_LOOP_ORIGIN
     }
     box      = boxNext;
     blockAct = blockActNext;
@@ -718,10 +701,7 @@ __global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
   __shared__ float sImg[STAGE ? 2 * kPitch * kFH : 1];
   __shared__ int sAct[2 * kWaves];
   if (!A.persistent) {
-    if constexpr (STAGE)
-      bp_tile_lds<R, PREC>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z, sImg, sAct);
-    else
-      bp_tile<R, PREC>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z);
+    bp_tile_lds<R, PREC, STAGE>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z, sImg, sAct);
     return;
   }
   const int nTiles  = A.tilesX * A.tilesY * A.tilesZ;
@@ -736,12 +716,8 @@ __global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
     const int tx = t % A.tilesX;
     const int ty = (t / A.tilesX) % A.tilesY;
     const int tz = A.tileZ0 + t / (A.tilesX * A.tilesY);
-    if constexpr (STAGE) {
-      __syncthreads(); // the previous tile's last LDS reads are done
-      bp_tile_lds<R, PREC>(A, tx, ty, tz, sImg, sAct);
-    } else {
-      bp_tile<R, PREC>(A, tx, ty, tz);
-    }
+    __syncthreads(); // the previous tile's last LDS reads (footprints, votes) are done
+    bp_tile_lds<R, PREC, STAGE>(A, tx, ty, tz, sImg, sAct);
   }
 }
 
@@ -1548,19 +1524,18 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   A.clip        = g.clip;
   const int R   = (int)g.runLength;
   const int stage = g.staging ? 1 : 0;
+  // both tap paths walk the same tiles and footprint boxes (the boxes fix
+  // the local coordinate origin)
   A.tilesX      = (int)g.L / R;
-  A.tilesY      = (int)g.L / (stage ? kLYs : kWave);
-  A.tilesZ      = (int)g.L / (stage ? kTileZ : kWaves);
+  A.tilesY      = (int)g.L / kLYs;
+  A.tilesZ      = (int)g.L / kTileZ;
   // the box table covers every z tile; a slab launch walks its own
   const int tilesZAll = A.tilesZ;
-  const int zTile     = stage ? kTileZ : kWaves;
-  A.tileZ0      = (int)c->slabZ0 / zTile;
-  A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / zTile;
+  A.tileZ0      = (int)c->slabZ0 / kTileZ;
+  A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / kTileZ;
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
-  A.boxes       = nullptr;
   A.boxSlots    = (int)g.nSlots;
-  A.boxBytes    = 0;
-  if (stage) {
+  {
     const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, tilesZAll, forceBoxes);
     if (rc != LH_OK)
       return rc;
@@ -1618,7 +1593,7 @@ static int graph_backproject(LhContext *c, uint32_t first, uint32_t count, uint3
       g.kernel == LH_KERNEL_FUSED ? (int)(per << 1 | (uint32_t)g.staging) : g.staging);
   auto it = c->graphCache.find(key);
   if (it == c->graphCache.end()) {
-    if (g.staging) {
+    {
       // allocate the box table outside the capture
       const int R  = (int)g.runLength;
       const int rc = box_storage(c, (long)(g.L / R) * (g.L / kLYs) * (g.L / kTileZ));
@@ -1644,9 +1619,8 @@ static int graph_backproject(LhContext *c, uint32_t first, uint32_t count, uint3
     it = c->graphCache.emplace(key, exec).first;
   }
   LH_HIP(hipGraphLaunch(it->second, c->stream));
-  if (g.staging)
-    for (uint32_t s = first; s < first + count; ++s)
-      c->boxR[s] = (int)g.runLength;
+  for (uint32_t s = first; s < first + count; ++s)
+    c->boxR[s] = (int)g.runLength;
   return LH_OK;
 }
 

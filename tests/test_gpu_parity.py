@@ -494,11 +494,13 @@ def test_full_detector_half_staged_vs_fp64(prec):
 def test_full_detector_phantom_vs_lolaopt_f32(L2, staging):
     """Full 1248x960 detector, phantom projections: at L=256 the staged kernel
     mixes LDS footprints with the L2 fallback (footprints up to 237 px wide),
-    at L=512 every footprint fits.  With 3 views the reference's own FP32
-    arithmetic (LolaOPT.c:131-173, restated in the oracle and pinned to the
-    compiled LolaOPT) is itself about 1e-4 of the range away from FP64; the
-    bound is 1e-4 or 1.5x the reference's distance, whichever is larger.  The
-    distance to the FP32 restatement is printed."""
+    at L=512 every footprint fits.  The volume is compared directly with the
+    reference's own FP32 arithmetic on the same input (LolaOPT.c:131-173,
+    restated in the oracle and pinned to the compiled LolaOPT) and with FP64:
+    max |d| <= 1e-4 of the range for both.  With 3 views LolaOPT's FP32 is
+    itself 0.8-0.95e-4 of the range from FP64; the kernel's coordinates,
+    local to each tile's footprint, keep its own distance to FP64 well below
+    that, so the two FP32 results stay within the bar."""
     geom, imgs, ranges, ref = _full_detector_case(L2, 3, 17, noise=False)
     v = lolahip.backproject_views(L2, geom.matrices, imgs, ranges, staging=staging)
     opt = orc.backproject_opt_f32(L2, voxel_size(L2), volume_origin(L2), geom.matrices, imgs,
@@ -509,7 +511,8 @@ def test_full_detector_phantom_vs_lolaopt_f32(L2, staging):
     err_ref = float(np.abs(opt - ref).max()) / rngv
     print(f"\nL={L2} staging={staging} phantom: max|d|/range vs LolaOPT FP32 {err_opt:.2e}; "
           f"vs FP64 {err:.2e} (LolaOPT FP32 vs FP64 {err_ref:.2e})")
-    assert err <= max(TOL, 1.5 * err_ref)
+    assert err_opt <= TOL
+    assert err <= TOL
 
 
 @pytest.mark.parametrize("staging", [True, False])
