@@ -198,6 +198,18 @@ int lhCancelUpload(LhContext *ctx);
  * context's stream (asynchronous). */
 int lhBackproject(LhContext *ctx, uint32_t firstSlot, uint32_t count);
 
+/* lhBackproject without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory while they run (code under HIP
+ * graph capture, tight serving loops).  The footprint boxes of each register
+ * pass go to a fixed 32 MiB table that the library holds per GPU instead of
+ * the context's table; the caller's slots, ranges and volume are used as by
+ * lhBackproject, with the same result bit for bit.  Launches are eager (no
+ * graph replay).  Returns LH_E_ARG when one pass needs more than the table's
+ * 4 Mi boxes (views per pass x tiles, tiles = (L/runLength) x (L/32) x
+ * (L/8): at most 128 views per pass at L=512 with 16-voxel runs).  Calls
+ * that overlap on one GPU share that table and race. */
+int lhBackprojectNoAlloc(LhContext *ctx, uint32_t firstSlot, uint32_t count);
+
 /* Runs `reps` times the backprojection of all slots [0, count) in passes of
  * cfg.viewsPerPass and returns, measured with HIP events on the context's
  * stream, the mean milliseconds per rep and per-launch statistics. */

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <cstdarg>
 #include <cstddef>
@@ -909,6 +910,41 @@ __global__ void line_ranges_kernel(int L, float rL, float oL, int W, int H,
 
 } // namespace
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// lhBackprojectNoAlloc keeps the footprint boxes of one register pass in this
+// file-scope array instead of the context's table (which lhBackproject
+// allocates on first use of a launch shape): 4 Mi boxes of 8 bytes (32 MiB),
+// one copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, resolved once
+// per device at context creation (scratch_boxes).  Calls on one GPU share the
+// array: calls that overlap on one GPU race.
+constexpr uint64_t kScratchBoxes = 4ull << 20;
+static __device__ __align__(256) int2 scratch[kScratchBoxes];
+
+namespace {
+std::mutex g_scratchMu;
+std::vector<int2 *> g_scratchPtr; // device address of `scratch`, by device ordinal
+
+// The device address of `scratch` on the current device `dev`.
+int scratch_boxes(int dev, int2 **out)
+{
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if ((int)g_scratchPtr.size() <= dev)
+    g_scratchPtr.resize(dev + 1, nullptr);
+  if (!g_scratchPtr[dev]) {
+    void *p             = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return lh_fail(LH_E_HIP, "scratch table of device %d: hipGetSymbolAddress: %s", dev,
+          hipGetErrorString(e));
+    g_scratchPtr[dev] = static_cast<int2 *>(p);
+  }
+  *out = g_scratchPtr[dev];
+  return LH_OK;
+}
+} // namespace
+
 // ---- context ---------------------------------------------------------------
 
 struct LhContext {
@@ -944,6 +980,7 @@ struct LhContext {
   std::vector<int> slotId;          // id held by each slot (0 = empty)
   uint32_t slabZ0 = 0, slabZ1 = 0;  // z-slab of the volume backprojected ([0, L) by default)
   int2 *dBoxes  = nullptr;          // staged kernel: footprint box per (slot, tile)
+  int2 *scratchBoxes = nullptr;     // device address of `scratch` on this context's device
   long boxTiles = 0;                // tiles per slot in dBoxes
   std::vector<int> boxR;            // run length the slot's boxes were built for (0 = none)
 };
@@ -1045,6 +1082,11 @@ static int create_buffers(LhContext *c)
 {
   const LhConfig *cfg = &c->cfg;
   LH_HIP(hipSetDevice(cfg->device));
+  {
+    const int rc = scratch_boxes(cfg->device, &c->scratchBoxes);
+    if (rc != LH_OK)
+      return rc;
+  }
   LH_HIP(hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, cfg->device));
   LH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -1502,7 +1544,12 @@ static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, int
   return LH_OK;
 }
 
-static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBoxes = false)
+// Launches the backprojection of slots [first, first+count).  extBoxes:
+// footprint boxes of exactly these slots, [tile][count], built by the caller
+// (the kernel then addresses views, slots and boxes relative to `first`);
+// nullptr: the context's box table, built here where needed.
+static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBoxes = false,
+    const int2 *extBoxes = nullptr)
 {
   const LhConfig &g = c->cfg;
   BpArgs A;
@@ -1535,7 +1582,14 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / kTileZ;
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
   A.boxSlots    = (int)g.nSlots;
-  {
+  if (extBoxes) {
+    A.views     = c->dViews + first;
+    A.slots     = c->dSlots + (uint64_t)first * c->slotBytes;
+    A.firstSlot = 0;
+    A.boxes     = extBoxes;
+    A.boxSlots  = (int)count;
+    A.boxBytes  = (uint32_t)((uint64_t)A.tilesX * A.tilesY * tilesZAll * count * sizeof(int2));
+  } else {
     const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, tilesZAll, forceBoxes);
     if (rc != LH_OK)
       return rc;
@@ -1811,6 +1865,48 @@ int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int rep
     return fail(LH_E_HIP, "lhTimedUploadHost: %s", hipGetErrorString(e));
   if (msPerRep)
     *msPerRep = (float)(total / reps);
+  return LH_OK;
+}
+
+int lhBackprojectNoAlloc(LhContext *c, uint32_t first, uint32_t count)
+{
+  if (!c)
+    return fail(LH_E_ARG, "lhBackprojectNoAlloc: null context");
+  const LhConfig &g = c->cfg;
+  if (count == 0)
+    return LH_OK;
+  if (first + count > g.nSlots)
+    return fail(LH_E_ARG, "lhBackprojectNoAlloc: slots [%u,%u) beyond %u", first, first + count,
+        g.nSlots);
+  for (uint32_t s = first; s < first + count; ++s)
+    if (c->slotId[s] == 0)
+      return fail(LH_E_STATE, "lhBackprojectNoAlloc: slot %u holds no view", s);
+  if (!c->scratchBoxes)
+    return fail(LH_E_STATE, "lhBackprojectNoAlloc: scratch table not resolved");
+  const uint32_t per   = g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 1u;
+  const int R          = (int)g.runLength;
+  const int tilesX     = (int)g.L / R, tilesY = (int)g.L / kLYs, tilesZ = (int)g.L / kTileZ;
+  const uint64_t tiles = (uint64_t)tilesX * tilesY * tilesZ;
+  const uint64_t need  = tiles * std::min(per, count);
+  if (need > kScratchBoxes)
+    return fail(LH_E_ARG,
+        "lhBackprojectNoAlloc: %u views per pass x %llu tiles need %llu footprint boxes; the "
+        "scratch table holds %llu (32 MiB): use fewer views per pass",
+        std::min(per, count), (unsigned long long)tiles, (unsigned long long)need,
+        (unsigned long long)kScratchBoxes);
+  LH_HIP(hipSetDevice(g.device));
+  // per pass: its boxes into the scratch table, then its launch (stream
+  // order keeps the next pass's boxes behind this pass's reads)
+  for (uint32_t s = first; s < first + count; s += per) {
+    const uint32_t n = std::min(per, first + count - s);
+    box_kernel<<<dim3((unsigned)((tiles + 255) / 256), n), 256, 0, c->stream>>>(c->dViews + s, 0,
+        (int)tiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H, (int)n,
+        c->scratchBoxes);
+    LH_HIP(hipGetLastError());
+    const int rc = launch_bp(c, s, n, false, c->scratchBoxes);
+    if (rc != LH_OK)
+      return rc;
+  }
   return LH_OK;
 }
 

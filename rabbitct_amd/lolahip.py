@@ -87,6 +87,8 @@ EXPORTS = (
     # host-memory reconstruction and the module's timing contract
     "lhQueryIdle", "lhHostAlloc", "lhHostFree", "lhReconstructHost", "lhTimedReconstructHost",
     "lolaHipQueryIdle", "lhDeviceInfo", "lhTimedUploadHost",
+    # allocation-free backprojection
+    "lhBackprojectNoAlloc",
 )
 
 _lib = None
@@ -181,6 +183,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lolaHipQueryIdle.argtypes = []
     lib.lhDeviceInfo.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.lhTimedUploadHost.argtypes = [P, P, u32, i32, ctypes.POINTER(f32)]
+    lib.lhBackprojectNoAlloc.argtypes = [P, u32, u32]
     _lib = lib
     return lib
 
@@ -383,6 +386,11 @@ class Context:
 
     def backproject(self, first, count):
         _check(self.lib.lhBackproject(self._h, first, count), "lhBackproject")
+
+    def backproject_no_alloc(self, first, count):
+        """lhBackprojectNoAlloc: lhBackproject with the footprint boxes in the
+        library's fixed per-GPU table (no device allocation)."""
+        _check(self.lib.lhBackprojectNoAlloc(self._h, first, count), "lhBackprojectNoAlloc")
 
     def synchronize(self):
         _check(self.lib.lhSynchronize(self._h), "lhSynchronize")
