@@ -115,14 +115,32 @@ def time_variant(args, ctx, n_views):
 
 
 def host_pipeline(args, geom, ranges, pinned, shape, prec, resident_vol):
+    """The best of host_pipeline_at over the tuned views-per-pass and half of
+    it (while a batch is backprojected, the next one streams into HBM through
+    the same 256 MB MALL that holds the projections being read, so smaller
+    batches can win here)."""
+    best = None
+    tried = {}
+    for vpp in sorted({shape["views_per_pass"], max(8, shape["views_per_pass"] // 2)},
+                      reverse=True):
+        r = host_pipeline_at(args, geom, ranges, pinned, {**shape, "views_per_pass": vpp},
+                             prec, resident_vol)
+        tried[vpp] = r["ms_per_step"]
+        if best is None or r["ms_per_step"] < best["ms_per_step"]:
+            best = r
+    best["ms_per_step_by_views_per_pass"] = tried
+    return best
+
+
+def host_pipeline_at(args, geom, ranges, pinned, shape, prec, resident_vol):
     """The reconstruction as a caller holding the views in host RAM runs it
     (lhTimedReconstructHost): all views in pinned host memory, per step the
-    H2D copies on the upload stream, staging on its own stream and the
-    graph-captured fused batches on the compute stream, double-buffered over
-    two slot sets of the tuned views-per-pass, timed with HIP events from the
+    H2D copies on the upload stream, one staging launch and the
+    graph-captured fused backprojection per batch on the compute stream,
+    double-buffered over two slot sets of the tuned views-per-pass (the first
+    and last batches shortened), timed with HIP events from the
     first copy to the last kernel.  Also times the copies alone (the PCIe
-    leg) and checks the pipeline's volume against the resident path's bit for
-    bit."""
+    leg) and compares the pipeline's volume with the resident path's."""
     vpp, run, staging = shape["views_per_pass"], shape["run_length"], shape["staging"]
     N = geom.n_views
     ids = np.arange(1, N + 1, dtype=np.int32)
@@ -137,7 +155,10 @@ def host_pipeline(args, geom, ranges, pinned, shape, prec, resident_vol):
         ctx.zero_volume()
         ctx.reconstruct_host(pinned, geom.matrices, ids)
         vol = ctx.get_volume()
-        same = bool(np.array_equal(vol.view(np.uint32), resident_vol.view(np.uint32)))
+        # the pipeline's batches (short first and last ones) are other passes
+        # than the resident path's, so the sums differ by FP32 rounding only
+        rng = float(resident_vol.max() - resident_vol.min())
+        diff = float(np.abs(vol - resident_vol).max()) / rng
         del vol
         if args.warmup:
             ctx.timed_reconstruct_host(pinned, geom.matrices, ids, args.warmup)
@@ -150,14 +171,16 @@ def host_pipeline(args, geom, ranges, pinned, shape, prec, resident_vol):
     out = {"ms_per_step": round(ms, 3), "gups": round(depth * args.L ** 2 * N / ms / 1e6, 2),
            "host_wall_ms_per_step": round(wall, 3),
            "h2d_only_ms": round(ms_h2d, 3), "h2d_gb_per_s": round(nbytes / ms_h2d / 1e6, 2),
-           "h2d_bytes_per_step": nbytes, "equal_to_resident_bitwise": same,
+           "h2d_bytes_per_step": nbytes, "max_diff_vs_resident": diff,
+           "batches": lolahip.host_batch_plan(N, vpp),
            "shape": {**shape, "slots": 2 * vpp, "graphs": True},
            "how": "lhTimedReconstructHost: views in pinned host memory (lhHostAlloc); H2D "
-                  "copies, staging and graph-captured fused batches on three streams, "
-                  "double-buffered slot sets; HIP events from the first copy to the last "
-                  "kernel"}
-    log(f"host pipeline ({prec}): {ms:.2f} ms/step = {out['gups']} GUPS; H2D alone "
-        f"{ms_h2d:.2f} ms ({out['h2d_gb_per_s']} GB/s); equal to resident: {same}")
+                  "copies into a two-batch device ring on the upload stream, one staging "
+                  "launch and one graph-captured fused backprojection per batch on the compute "
+                  "stream, double-buffered slot sets; HIP events from the first copy to the "
+                  "last kernel"}
+    log(f"host pipeline ({prec}, {vpp} views per pass): {ms:.2f} ms/step = {out['gups']} GUPS; H2D alone "
+        f"{ms_h2d:.2f} ms ({out['h2d_gb_per_s']} GB/s); max diff vs resident {diff:.2e} of range")
     return out
 
 

@@ -240,15 +240,22 @@ int lhHostFree(void *p);
  * LolaOPT.c:111-117, plus the backprojection): views [0, count) of images
  * (count x H x W floats, host; pinned memory from lhHostAlloc makes the
  * copies asynchronous), matrices (count x 12) and ids (count, 1-based) are
- * uploaded on the context's upload stream, staged into the slots on a staging
- * stream and backprojected (+=) on the compute stream, in batches of
- * viewsPerPass views (fused; 16 views per batch for plain launches) over
- * nSlots / batch slot sets: batch k+1 crosses PCIe and is staged while batch k
- * is backprojected.  With lhSetGraphs on, each batch replays a captured HIP
- * graph.  Asynchronous: lhSynchronize waits for the end.  The host arrays must
- * stay unchanged until then.  No counterpart in the reference. */
+ * copied into a device ring of two batches on the context's upload stream and
+ * staged into the slots (one launch per batch) and backprojected (+=) on the
+ * compute stream, in batches of viewsPerPass views (fused; 16 for plain
+ * launches; the first and the last two batches a quarter as long, so the
+ * pipeline fills and drains quickly) over nSlots / batch slot sets: batch k+1
+ * crosses PCIe while batch k is backprojected.  With lhSetGraphs on, each
+ * batch's backprojection replays a captured HIP graph.  Asynchronous:
+ * lhSynchronize waits for the end.  The host arrays must stay unchanged until
+ * then.  No counterpart in the reference. */
 int lhReconstructHost(LhContext *ctx, const float *images, const double *matrices,
     const int *ids, uint32_t count);
+
+/* The batch sizes lhReconstructHost uses for `count` views in batches of
+ * `batch` (the first and the last two shortened): returns the number of
+ * batches and writes up to maxSizes of them to sizes (may be null). */
+uint32_t lhHostBatchPlan(uint32_t count, uint32_t batch, uint32_t *sizes, uint32_t maxSizes);
 
 /* lhReconstructHost `reps` times, each from idle streams, timed with HIP
  * events from before the first H2D copy to after the last backprojection:
@@ -257,10 +264,10 @@ int lhTimedReconstructHost(LhContext *ctx, const float *images, const double *ma
     const int *ids, uint32_t count, int reps, float *msPerRep);
 
 /* The PCIe leg of lhReconstructHost alone: the H2D copies of views
- * [0, count) of images (host, count x H x W floats) into the context's device
- * upload buffers on the upload stream, back to back, `reps` times; mean
- * milliseconds per repetition (HIP events).  No counterpart in the
- * reference. */
+ * [0, count) of images (host, count x H x W floats) into its device ring, one
+ * copy per batch as lhReconstructHost makes them, back to back on the upload
+ * stream, `reps` times; mean milliseconds per repetition (HIP events).  No
+ * counterpart in the reference. */
 int lhTimedUploadHost(LhContext *ctx, const float *images, uint32_t count, int reps,
     float *msPerRep);
 

@@ -731,8 +731,11 @@ __global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
 // image, one per workgroup of a 256-workgroup grid ~5 us).
 constexpr int kMaxAbsThreads = 256;
 __global__ __launch_bounds__(kMaxAbsThreads) void maxabs_kernel(const float *__restrict__ img,
-    uint32_t n, uint32_t *out)
+    uint32_t n, uint32_t *out, uint64_t imgStride = 0)
 {
+  // a batch of images: image blockIdx.y at img + blockIdx.y * imgStride
+  img += (uint64_t)blockIdx.y * imgStride;
+  out += blockIdx.y;
   __shared__ uint32_t sMax[kMaxAbsThreads / kWave];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   const uint32_t n4  = ((uintptr_t)img & 15) == 0 ? n / 4 : 0;
@@ -794,15 +797,14 @@ __device__ __forceinline__ uint16_t to_f16(float f)
 // the difference's own rounding.  Block (0,0) thread 0 also applies the storage scale
 // to the slot's view parameters.
 template <int PREC>
-__global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
-    uint32_t rowBytes, const uint32_t *maxbits, ViewParams vp, ViewParams *out)
+__device__ __forceinline__ void stage_element(const float *__restrict__ img, int W, int H,
+    uint8_t *slot, uint32_t rowBytes, const uint32_t *maxbits, ViewParams vp, ViewParams *out,
+    int ei, int ej, bool leader)
 {
   // only FP16 storage is scaled (maxabs_kernel filled *maxbits)
   const int m    = PREC == LH_PREC_F16 ? storage_exponent(PREC, *maxbits) : 0;
   const float is = ldexpf(1.0f, -2 * m);
-  const int ei   = blockIdx.x * blockDim.x + threadIdx.x; // element column
-  const int ej   = blockIdx.y;                            // element row
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+  if (leader) {
     if (PREC != LH_PREC_F32) {
       const double k = ldexp(1.0, -m);
       for (int c = 0; c < 12; ++c)
@@ -832,6 +834,29 @@ __global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_
     const uint32_t q = (uint32_t)h0 | ((uint32_t)h1 << 16);
     reinterpret_cast<uint32_t *>(slot + (size_t)ej * rowBytes)[ei] = q;
   }
+}
+
+template <int PREC>
+__global__ void stage_kernel(const float *__restrict__ img, int W, int H, uint8_t *slot,
+    uint32_t rowBytes, const uint32_t *maxbits, ViewParams vp, ViewParams *out)
+{
+  stage_element<PREC>(img, W, H, slot, rowBytes, maxbits, vp, out,
+      blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y,
+      blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0);
+}
+
+// A batch of views in one launch: view z = blockIdx.z from img + z * imgStride
+// into slot z of `slots`, with view parameters vpIn[z] (the host-computed
+// FP64 matrix and steps) and max-abs cell maxbits[z].
+template <int PREC>
+__global__ void stage_batch_kernel(const float *__restrict__ img, uint64_t imgStride, int W,
+    int H, uint8_t *slots, uint64_t slotBytes, uint32_t rowBytes, const uint32_t *maxbits,
+    const ViewParams *__restrict__ vpIn, ViewParams *out)
+{
+  const int z = blockIdx.z;
+  stage_element<PREC>(img + (uint64_t)z * imgStride, W, H, slots + (uint64_t)z * slotBytes,
+      rowBytes, maxbits + z, vpIn[z], out + z, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y,
+      blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0);
 }
 
 // vol *= s (float4 body, scalar tail)
@@ -961,13 +986,17 @@ struct LhContext {
   int nRing = 0, ringNext = 0, ringAcquired = -1;
   std::vector<float *> hRing, dRing;
   std::vector<hipEvent_t> copyDone, stageDone;
-  // host-memory reconstruction (lhReconstructHost): staging kernels run on
-  // stageStream, beside the backprojection of the previous batch; per slot
-  // set, an event after its last staging kernel and one after the
-  // backprojection that read it
-  hipStream_t stageStream = nullptr;
-  std::vector<hipEvent_t> setStaged, setDone;
-  std::vector<bool> setUsed;
+  // host-memory reconstruction (lhReconstructHost): a device ring of two
+  // halves of `hostBatch` W*H images and view parameters, filled by H2D copies
+  // on upStream and staged into the slots by one launch per batch on the
+  // compute stream; per half, an event after its copies and one after the
+  // staging launch that read it
+  uint32_t hostBatch = 0;
+  float *dHostRing   = nullptr;
+  ViewParams *dHostParams = nullptr;
+  ViewParams *hHostParams = nullptr; // pinned, one per view of the call
+  uint32_t hHostParamsCap = 0;
+  hipEvent_t halfCopied[2] = { nullptr, nullptr }, halfFree[2] = { nullptr, nullptr };
   // captured backprojection batches, by (first slot, count, kernel, run, staging)
   bool graphs = false;
   std::map<std::tuple<uint32_t, uint32_t, int, int, int>, hipGraphExec_t> graphCache;
@@ -1113,7 +1142,6 @@ static int create_buffers(LhContext *c)
   // one ring entry per view of a register pass (2..32), so a whole batch can
   // be in flight while the previous one is backprojected
   LH_HIP(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
-  LH_HIP(hipStreamCreateWithFlags(&c->stageStream, hipStreamNonBlocking));
   c->nRing = (int)std::min<uint32_t>(32u, std::max<uint32_t>(2u, cfg->viewsPerPass));
   c->hRing.assign(c->nRing, nullptr);
   c->dRing.assign(c->nRing, nullptr);
@@ -1138,17 +1166,23 @@ int lhDestroy(LhContext *c)
   if (!c)
     return LH_OK;
   // every stream drains before the buffers its work reads or writes go away
-  for (hipStream_t st : { c->stream, c->upStream, c->stageStream })
+  for (hipStream_t st : { c->stream, c->upStream })
     if (st)
       (void)hipStreamSynchronize(st);
   (void)hipFree(c->dVol);
   (void)hipFree(c->dRanges);
   (void)hipFree(c->dViews);
   (void)hipFree(c->dSlots);
-  for (hipEvent_t e : c->setStaged)
-    (void)hipEventDestroy(e);
-  for (hipEvent_t e : c->setDone)
-    (void)hipEventDestroy(e);
+  for (int h = 0; h < 2; ++h) {
+    if (c->halfCopied[h])
+      (void)hipEventDestroy(c->halfCopied[h]);
+    if (c->halfFree[h])
+      (void)hipEventDestroy(c->halfFree[h]);
+  }
+  (void)hipFree(c->dHostRing);
+  (void)hipFree(c->dHostParams);
+  if (c->hHostParams)
+    (void)hipHostFree(c->hHostParams);
   for (auto &kv : c->graphCache)
     (void)hipGraphExecDestroy(kv.second);
   (void)hipFree(c->dMax);
@@ -1164,8 +1198,6 @@ int lhDestroy(LhContext *c)
   }
   if (c->upStream)
     (void)hipStreamDestroy(c->upStream);
-  if (c->stageStream)
-    (void)hipStreamDestroy(c->stageStream);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1452,7 +1484,7 @@ int lhQueryIdle(LhContext *c)
   if (!c)
     return fail(LH_E_ARG, "lhQueryIdle: null context");
   LH_HIP(hipSetDevice(c->cfg.device));
-  for (hipStream_t st : { c->stream, c->upStream, c->stageStream }) {
+  for (hipStream_t st : { c->stream, c->upStream }) {
     const hipError_t e = hipStreamQuery(st);
     if (e == hipErrorNotReady)
       return 0;
@@ -1702,24 +1734,82 @@ int lhBackproject(LhContext *c, uint32_t first, uint32_t count)
   return LH_OK;
 }
 
-// One view from host memory into slot `slot`: the H2D copy into the next
-// device ring buffer on upStream (after the staging kernel that last read that
-// buffer), then its staging on stageStream (after the copy).
-static int upload_host_view(LhContext *c, uint32_t slot, int id, const double *matrix,
-    const float *host)
+// View parameters of `matrix` for view `id` (unscaled: the staging applies
+// the FP16 storage scale).
+static ViewParams view_params_of(const LhConfig &g, int id, const double *matrix)
 {
-  const int k        = c->ringNext;
-  const size_t bytes = (size_t)c->cfg.W * c->cfg.H * sizeof(float);
-  LH_HIP(hipStreamWaitEvent(c->upStream, c->stageDone[k], 0));
-  LH_HIP(hipMemcpyAsync(c->dRing[k], host, bytes, hipMemcpyHostToDevice, c->upStream));
-  LH_HIP(hipEventRecord(c->copyDone[k], c->upStream));
-  LH_HIP(hipStreamWaitEvent(c->stageStream, c->copyDone[k], 0));
-  const int rc = load_view_common(c, slot, id, matrix, c->dRing[k], c->stageStream);
-  if (rc != LH_OK)
-    return rc;
-  LH_HIP(hipEventRecord(c->stageDone[k], c->stageStream));
-  c->ringNext = (k + 1) % c->nRing;
+  ViewParams vp;
+  std::memset(&vp, 0, sizeof vp);
+  for (int k = 0; k < 12; ++k)
+    vp.a[k] = matrix[k];
+  vp.du    = (float)(matrix[0] * (double)g.R_L);
+  vp.dv    = (float)(matrix[1] * (double)g.R_L);
+  vp.dw    = (float)(matrix[2] * (double)g.R_L);
+  vp.id    = id;
+  vp.scale = 1.0f;
+  return vp;
+}
+
+// Batch sizes of lhReconstructHost: batches of `batch` views, the first and
+// the last two `ramp` views long, so the pipeline fills after one short upload
+// and drains after one short backprojection (the copies and the kernels run
+// at about the same rate).
+static std::vector<uint32_t> host_batches(uint32_t count, uint32_t batch)
+{
+  std::vector<uint32_t> out;
+  const uint32_t ramp = std::max(1u, batch / 4);
+  uint32_t rem        = count;
+  if (count <= 2 * batch || ramp == batch) {
+    for (; rem > 0; rem -= std::min(batch, rem))
+      out.push_back(std::min(batch, rem));
+    return out;
+  }
+  out.push_back(ramp);
+  rem -= ramp;
+  while (rem > batch + ramp) {
+    out.push_back(batch);
+    rem -= batch;
+  }
+  if (rem > ramp)
+    out.push_back(rem - ramp);
+  out.push_back(std::min(ramp, rem));
+  return out;
+}
+
+// The host pipeline's device ring (two halves of `batch` images) and its
+// parameter buffers, kept for later calls of the same batch size.
+static int host_ring(LhContext *c, uint32_t batch)
+{
+  if (c->hostBatch == batch)
+    return LH_OK;
+  const size_t px = (size_t)c->cfg.W * c->cfg.H;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->upStream);
+  (void)hipFree(c->dHostRing);
+  (void)hipFree(c->dHostParams);
+  c->dHostRing   = nullptr;
+  c->dHostParams = nullptr;
+  c->hostBatch   = 0;
+  LH_HIP(hipMalloc(&c->dHostRing, 2 * (size_t)batch * px * sizeof(float)));
+  LH_HIP(hipMalloc(&c->dHostParams, 2 * (size_t)batch * sizeof(ViewParams)));
+  for (int h = 0; h < 2; ++h) {
+    if (!c->halfCopied[h])
+      LH_HIP(hipEventCreateWithFlags(&c->halfCopied[h], hipEventDisableTiming));
+    if (!c->halfFree[h])
+      LH_HIP(hipEventCreateWithFlags(&c->halfFree[h], hipEventDisableTiming));
+  }
+  c->hostBatch = batch;
   return LH_OK;
+}
+
+uint32_t lhHostBatchPlan(uint32_t count, uint32_t batch, uint32_t *sizes, uint32_t maxSizes)
+{
+  if (count == 0 || batch == 0)
+    return 0;
+  const std::vector<uint32_t> p = host_batches(count, batch);
+  for (size_t k = 0; sizes && k < p.size() && k < maxSizes; ++k)
+    sizes[k] = p[k];
+  return (uint32_t)p.size();
 }
 
 int lhReconstructHost(LhContext *c, const float *images, const double *matrices, const int *ids,
@@ -1727,49 +1817,91 @@ int lhReconstructHost(LhContext *c, const float *images, const double *matrices,
 {
   if (!c || !images || !matrices || !ids || count == 0)
     return fail(LH_E_ARG, "lhReconstructHost: bad arguments");
-  if (c->ringAcquired >= 0)
-    return fail(LH_E_STATE, "lhReconstructHost: an upload buffer is acquired");
   const LhConfig &g    = c->cfg;
   const uint32_t batch = std::min(g.nSlots,
       g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 16u);
   const uint32_t nsets = g.nSlots / batch;
-  LH_HIP(hipSetDevice(g.device));
-  while (c->setDone.size() < nsets) {
-    hipEvent_t a = nullptr, b = nullptr;
-    LH_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-    if (hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess) {
-      (void)hipEventDestroy(a);
-      return fail(LH_E_HIP, "lhReconstructHost: event creation failed");
-    }
-    c->setStaged.push_back(a);
-    c->setDone.push_back(b);
-    c->setUsed.push_back(false);
-  }
-  // staging waits for everything already enqueued on the compute stream
-  // (earlier backprojections may still read the slots)
-  LH_HIP(hipEventRecord(c->setStaged[0], c->stream));
-  LH_HIP(hipStreamWaitEvent(c->stageStream, c->setStaged[0], 0));
+  for (uint32_t v = 0; v < count; ++v)
+    if (ids[v] < 1 || (uint32_t)ids[v] > g.nViews)
+      return fail(LH_E_ARG, "lhReconstructHost: view id %d out of range", ids[v]);
   const size_t px = (size_t)g.W * g.H;
-  uint32_t k = 0;
-  for (uint32_t v0 = 0; v0 < count; v0 += batch, ++k) {
-    const uint32_t set = k % nsets, base = set * batch, n = std::min(batch, count - v0);
-    // the slot set's previous batch must have been backprojected
-    if (c->setUsed[set])
-      LH_HIP(hipStreamWaitEvent(c->stageStream, c->setDone[set], 0));
-    for (uint32_t j = 0; j < n; ++j) {
-      const uint32_t v = v0 + j;
-      const int rc = upload_host_view(c, base + j, ids[v], matrices + 12 * (size_t)v,
-          images + (size_t)v * px);
-      if (rc != LH_OK)
-        return rc;
+  LH_HIP(hipSetDevice(g.device));
+  {
+    const int rc = host_ring(c, batch);
+    if (rc != LH_OK)
+      return rc;
+  }
+  if (c->hHostParamsCap < count) {
+    // the previous call's parameter copies must have read the old buffer
+    LH_HIP(hipStreamSynchronize(c->upStream));
+    if (c->hHostParams)
+      (void)hipHostFree(c->hHostParams);
+    c->hHostParams    = nullptr;
+    c->hHostParamsCap = 0;
+    LH_HIP(hipHostMalloc((void **)&c->hHostParams, count * sizeof(ViewParams),
+        hipHostMallocDefault));
+    c->hHostParamsCap = count;
+  } else {
+    LH_HIP(hipStreamSynchronize(c->upStream)); // the buffer is rewritten below
+  }
+  for (uint32_t v = 0; v < count; ++v)
+    c->hHostParams[v] = view_params_of(g, ids[v], matrices + 12 * (size_t)v);
+  // the first copies wait for the compute stream's earlier work (a previous
+  // call's staging may still read the ring)
+  LH_HIP(hipEventRecord(c->halfFree[0], c->stream));
+  LH_HIP(hipEventRecord(c->halfFree[1], c->stream));
+  const std::vector<uint32_t> sizes = host_batches(count, batch);
+  const int cols = g.precision == LH_PREC_F32 ? (int)g.W + 4 : (int)g.W + 3;
+  uint32_t v0 = 0;
+  for (size_t k = 0; k < sizes.size(); ++k) {
+    const uint32_t n = sizes[k], half = (uint32_t)(k & 1);
+    const uint32_t base = (uint32_t)(k % nsets) * batch;
+    float *ring         = c->dHostRing + (size_t)half * batch * px;
+    ViewParams *params  = c->dHostParams + (size_t)half * batch;
+    // upload stream: the half's previous staging has read it
+    LH_HIP(hipStreamWaitEvent(c->upStream, c->halfFree[half], 0));
+    // the batch's images are contiguous on both sides: one copy
+    LH_HIP(hipMemcpyAsync(ring, images + (size_t)v0 * px, (size_t)n * px * sizeof(float),
+        hipMemcpyHostToDevice, c->upStream));
+    LH_HIP(hipMemcpyAsync(params, c->hHostParams + v0, n * sizeof(ViewParams),
+        hipMemcpyHostToDevice, c->upStream));
+    LH_HIP(hipEventRecord(c->halfCopied[half], c->upStream));
+    // compute stream: stage the batch (stream order puts it after the
+    // backprojection that last read these slots), then backproject it
+    LH_HIP(hipStreamWaitEvent(c->stream, c->halfCopied[half], 0));
+    if (g.precision == LH_PREC_F16) {
+      LH_HIP(hipMemsetAsync(c->dMax + base, 0, n * sizeof(uint32_t), c->stream));
+      const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(
+          (uint32_t)((px / 4 + 4 * kMaxAbsThreads - 1) / (4 * kMaxAbsThreads)), 256u));
+      maxabs_kernel<<<dim3(blocks, n), kMaxAbsThreads, 0, c->stream>>>(ring, (uint32_t)px,
+          c->dMax + base, px);
     }
-    LH_HIP(hipEventRecord(c->setStaged[set], c->stageStream));
-    LH_HIP(hipStreamWaitEvent(c->stream, c->setStaged[set], 0));
+    const dim3 grid((cols + 255) / 256, g.H + 4, n);
+    uint8_t *slots = c->dSlots + (uint64_t)base * c->slotBytes;
+    switch (g.precision) {
+    case LH_PREC_F32:
+      stage_batch_kernel<LH_PREC_F32><<<grid, 256, 0, c->stream>>>(ring, px, g.W, g.H, slots,
+          c->slotBytes, c->rowBytes, c->dMax + base, params, c->dViews + base);
+      break;
+    case LH_PREC_BF16:
+      stage_batch_kernel<LH_PREC_BF16><<<grid, 256, 0, c->stream>>>(ring, px, g.W, g.H, slots,
+          c->slotBytes, c->rowBytes, c->dMax + base, params, c->dViews + base);
+      break;
+    default:
+      stage_batch_kernel<LH_PREC_F16><<<grid, 256, 0, c->stream>>>(ring, px, g.W, g.H, slots,
+          c->slotBytes, c->rowBytes, c->dMax + base, params, c->dViews + base);
+      break;
+    }
+    LH_HIP(hipGetLastError());
+    LH_HIP(hipEventRecord(c->halfFree[half], c->stream));
+    for (uint32_t j = 0; j < n; ++j) {
+      c->slotId[base + j] = ids[v0 + j];
+      c->boxR[base + j]   = 0;
+    }
     const int rc = lhBackproject(c, base, n);
     if (rc != LH_OK)
       return rc;
-    LH_HIP(hipEventRecord(c->setDone[set], c->stream));
-    c->setUsed[set] = true;
+    v0 += n;
   }
   return LH_OK;
 }
@@ -1791,8 +1923,6 @@ int lhTimedReconstructHost(LhContext *c, const float *images, const double *matr
   for (int rep = 0; rep < reps && rc == LH_OK; ++rep) {
     // each repetition starts on idle streams, at its first copy
     hipError_t e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess)
-      e = hipStreamSynchronize(c->stageStream);
     if (e == hipSuccess)
       e = hipStreamSynchronize(c->upStream);
     if (e == hipSuccess)
@@ -1827,10 +1957,16 @@ int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int rep
 {
   if (!c || !images || count == 0 || reps < 1)
     return fail(LH_E_ARG, "lhTimedUploadHost: bad arguments");
-  if (c->ringAcquired >= 0)
-    return fail(LH_E_STATE, "lhTimedUploadHost: an upload buffer is acquired");
-  const size_t px = (size_t)c->cfg.W * c->cfg.H;
-  LH_HIP(hipSetDevice(c->cfg.device));
+  const LhConfig &g    = c->cfg;
+  const uint32_t batch = std::min(g.nSlots,
+      g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 16u);
+  const size_t px = (size_t)g.W * g.H;
+  LH_HIP(hipSetDevice(g.device));
+  {
+    const int rc = host_ring(c, batch);
+    if (rc != LH_OK)
+      return rc;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   LH_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1839,16 +1975,18 @@ int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int rep
   }
   hipError_t e = hipSuccess;
   double total = 0.0;
+  const std::vector<uint32_t> sizes = host_batches(count, batch);
   for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
-    // the ring buffers are overwritten: every stream that may read them drains first
-    for (hipStream_t st : { c->stream, c->stageStream, c->upStream })
+    // the ring is overwritten: the compute stream, which may read it, drains first
+    for (hipStream_t st : { c->stream, c->upStream })
       if (e == hipSuccess)
         e = hipStreamSynchronize(st);
     if (e == hipSuccess)
       e = hipEventRecord(e0, c->upStream);
-    for (uint32_t v = 0; v < count && e == hipSuccess; ++v)
-      e = hipMemcpyAsync(c->dRing[v % (uint32_t)c->nRing], images + v * px, px * sizeof(float),
-          hipMemcpyHostToDevice, c->upStream);
+    uint32_t v0 = 0;
+    for (size_t k = 0; k < sizes.size() && e == hipSuccess; v0 += sizes[k], ++k)
+      e = hipMemcpyAsync(c->dHostRing + (k & 1) * (size_t)batch * px, images + (size_t)v0 * px,
+          (size_t)sizes[k] * px * sizeof(float), hipMemcpyHostToDevice, c->upStream);
     if (e == hipSuccess)
       e = hipEventRecord(e1, c->upStream);
     if (e == hipSuccess)

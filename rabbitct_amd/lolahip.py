@@ -88,7 +88,7 @@ EXPORTS = (
     "lhQueryIdle", "lhHostAlloc", "lhHostFree", "lhReconstructHost", "lhTimedReconstructHost",
     "lolaHipQueryIdle", "lhDeviceInfo", "lhTimedUploadHost",
     # allocation-free backprojection
-    "lhBackprojectNoAlloc",
+    "lhBackprojectNoAlloc", "lhHostBatchPlan",
 )
 
 _lib = None
@@ -184,6 +184,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhDeviceInfo.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.lhTimedUploadHost.argtypes = [P, P, u32, i32, ctypes.POINTER(f32)]
     lib.lhBackprojectNoAlloc.argtypes = [P, u32, u32]
+    lib.lhHostBatchPlan.argtypes = [u32, u32, P, u32]
+    lib.lhHostBatchPlan.restype = u32
     _lib = lib
     return lib
 
@@ -282,6 +284,15 @@ def device_info(device: int = 0) -> dict:
     _check(load_library().lhDeviceInfo(device, ctypes.byref(cu), ctypes.byref(khz)),
            "lhDeviceInfo")
     return {"compute_units": cu.value, "clock_mhz": khz.value / 1000.0}
+
+
+def host_batch_plan(count: int, batch: int) -> list:
+    """Batch sizes of lhReconstructHost for `count` views (lhHostBatchPlan)."""
+    lib = load_library()
+    n = int(lib.lhHostBatchPlan(count, batch, None, 0))
+    out = np.zeros(max(n, 1), np.uint32)
+    lib.lhHostBatchPlan(count, batch, _p(out), n)
+    return [int(v) for v in out[:n]]
 
 
 def module_query_idle() -> int:

@@ -21,7 +21,8 @@ L = 128
 def test_library_exports_every_declared_symbol():
     lib = lolahip.load_library()
     header = open(os.path.join(ROOT, "include", "lolahip.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|void|uint64_t|const char \*)\s*\**\s*(l[a-zA-Z0-9]+)\(",
+    declared = set(re.findall(r"^\s*(?:int|void|uint32_t|uint64_t|const char \*)\s*\**\s*"
+                              r"(l[a-zA-Z0-9]+)\(",
                               header, re.M))
     assert declared == set(lolahip.EXPORTS)
     for name in declared:
@@ -208,3 +209,17 @@ def test_parker_weights_partition_of_unity():
             i2 = g.width - 1 - i
             if 0 <= n2 < g.n_views and abs(n2 * db - b2) < 1e-9 + db / 2:
                 assert w[n, i] + w[n2, i2] == pytest.approx(1.0, abs=0.02)
+
+
+def test_host_batch_plan_covers_the_views_with_short_ends():
+    """lhReconstructHost's batches: they sum to the view count, none exceeds
+    the batch size, and beyond two batches the first and the last two are a
+    quarter batch long (the pipeline fills after one short upload and drains
+    after one short backprojection)."""
+    from rabbitct_amd import lolahip
+    for count, batch in ((496, 64), (62, 16), (62, 8), (62, 64), (10, 4), (1, 1), (129, 64)):
+        plan = lolahip.host_batch_plan(count, batch)
+        assert sum(plan) == count and max(plan) <= batch and min(plan) >= 1
+        if count > 2 * batch and batch >= 4:
+            assert plan[0] == batch // 4 and plan[-1] == batch // 4
+    assert lolahip.host_batch_plan(496, 64) == [16] + [64] * 7 + [16, 16]
