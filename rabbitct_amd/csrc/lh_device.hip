@@ -265,8 +265,11 @@ __device__ __forceinline__ float l2_update(__amdgpu_buffer_rsrc_t rs, uint32_t b
 // at L=512 with R=16: FP32 fastest with groups of 16, that is one fence at
 // the end of the run, 47.4 against 48.5 ms per step with groups of 2 or 4;
 // the half-precision kernels fastest with 4)
+#ifndef LH_FENCE32
+#define LH_FENCE32 16
+#endif
 template <int PREC>
-constexpr int kFence = PREC == LH_PREC_F32 ? 16 : 4;
+constexpr int kFence = PREC == LH_PREC_F32 ? LH_FENCE32 : 4;
 // updates per scheduling group of the L2-gather fallback, by run length
 // (measured at L=256, where most footprints take it: with R=16 one voxel's
 // loads at a time is fastest, with R=32 four)
@@ -277,6 +280,9 @@ constexpr int kFenceL2 = R == 32 ? 4 : 1;
 #endif
 #ifndef LH_FH
 #define LH_FH 39
+#endif
+#ifndef LH_OCC16
+#define LH_OCC16 4
 #endif
 constexpr int kFW    = 128;      // widest footprint staged (pixels)
 constexpr int kPitch = LH_PITCH; // LDS row pitch (dwords) >= kFW; ds_read2 offsets stay <= 255
@@ -495,9 +501,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     return A.clip ? A.ranges[(uint64_t)(view_params(A, A.firstSlot + v).id - 1) * LL + line]
                   : (L << 16);
   };
-  auto lane_active = [&](int rr) -> bool {
-    const int start = (int)(short)(rr & 0xffff), stop = rr >> 16;
-    return max(start - x0, 0) < min(stop - x0, R);
+  // the lane's run [lo, hi) within its line's range [start, stop)
+  auto decode = [&](int rr, int &lo, int &hi) {
+    lo = max((int)(short)(rr & 0xffff) - x0, 0);
+    hi = min((rr >> 16) - x0, R);
   };
 
   float acc[R];
@@ -506,11 +513,13 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     acc[r] = 0.0f;
   bool touched = false;
 
-  // prologue: view 0 activity and footprint
-  int rr_cur = range_of(0);
+  // prologue: view 0 activity and footprint.  Each range is decoded once,
+  // for the activity vote one view ahead, and kept for its own view.
+  int lo_c, hi_c;
+  decode(range_of(0), lo_c, hi_c);
   int rr_nxt = n > 1 ? range_of(1) : 0;
   // the vote runs on the whole wave, only the store is lane 0's
-  const bool act0 = __any(lane_active(rr_cur));
+  const bool act0 = __any(lo_c < hi_c);
   if (lane == 0)
     sAct[wv] = act0 ? 1 : 0;
   __syncthreads();
@@ -528,14 +537,14 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     const bool hasNext  = v + 1 < n;
     // footprint(v) and the range loads have landed; publish activity of v+1
     __builtin_amdgcn_s_waitcnt(0);
-    const bool actNext = hasNext && __any(lane_active(rr_nxt));
+    int lo_n = 0, hi_n = 0;
+    decode(rr_nxt, lo_n, hi_n);
+    const bool actNext = hasNext && __any(lo_n < hi_n);
     if (lane == 0)
       sAct[((v + 1) & 1) * kWaves + wv] = actNext ? 1 : 0;
     __syncthreads();
     const int *an = sAct + ((v + 1) & 1) * kWaves;
     const bool blockActNext = __builtin_amdgcn_readfirstlane(an[0] | an[1] | an[2] | an[3]) != 0;
-    const int rr   = rr_cur;
-    rr_cur         = rr_nxt;
     if (v + 2 < n)
       rr_nxt = range_of(v + 2);
     StageBox boxNext = box;
@@ -549,10 +558,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
             sImg + ((v + 1) & 1) * kPitch * kFH, wv, laneOff, loHalf);
     }
 
-    const int start   = (int)(short)(rr & 0xffff);
-    const int stop    = rr >> 16;
-    const int rlo     = max(start - x0, 0);
-    const int rhi     = min(stop - x0, R);
+    const int rlo     = lo_c;
+    const int rhi     = hi_c;
+    lo_c              = lo_n;
+    hi_c              = hi_n;
     const bool active = rlo < rhi;
     if (blockAct && __any(active)) {
       touched |= active;
@@ -560,9 +569,12 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       // u' = u - iu0 w, v' = v - iv0 w in FP64, then rounded once to FP32; the
       // run's voxels step from it in FP32, one rounding each, so nothing
       // drifts along x at any L
-      const double U0 = fma(P.a[0], wx0, fma(P.a[3], wy, fma(P.a[6], wz, P.a[9])));
-      const double V0 = fma(P.a[1], wx0, fma(P.a[4], wy, fma(P.a[7], wz, P.a[10])));
-      const double W0 = fma(P.a[2], wx0, fma(P.a[5], wy, fma(P.a[8], wz, P.a[11])));
+      // (a6 wz first as a product: each instruction then reads one scalar
+      // operand, which the vector ALU allows without copying the matrix into
+      // vector registers)
+      const double U0 = fma(P.a[0], wx0, fma(P.a[3], wy, P.a[6] * wz)) + P.a[9];
+      const double V0 = fma(P.a[1], wx0, fma(P.a[4], wy, P.a[7] * wz)) + P.a[10];
+      const double W0 = fma(P.a[2], wx0, fma(P.a[5], wy, P.a[8] * wz)) + P.a[11];
       float u0 = (float)fma(-(double)box.iu0, W0, U0);
       float v0 = (float)fma(-(double)box.iv0, W0, V0);
       float w0 = (float)W0;
@@ -696,7 +708,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 // registers, which their L2-gather fallback needs to keep four voxels' taps in
 // flight; the launcher picks OCC = 4 for them where footprints fit the LDS
 // window (small tiles).
-template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : 4)>
+template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : LH_OCC16)>
 __global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
 {
   __shared__ float sImg[STAGE ? 2 * kPitch * kFH : 1];
@@ -1016,7 +1028,7 @@ struct LhContext {
 
 // Grid of one launch: one workgroup per tile, or for the persistent form as
 // many workgroups as the chip keeps resident (occupancy x CUs).
-template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : 4)>
+template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : LH_OCC16)>
 static void launch_shape(LhContext *c, const BpArgs &A)
 {
   static int perCU = 0;
