@@ -69,6 +69,7 @@ def make_context(args, geom, phantom, ranges, kernel, prec, vpp, host_out=None):
                           run_length=args.run)
     if args.slab:
         ctx.set_slab(*args.slab)
+    ctx.set_tile(args.tile_y)
     ctx.set_ranges(ranges)
     generate_into(ctx, geom, phantom, out=host_out)
     ctx.synchronize()
@@ -149,6 +150,7 @@ def host_pipeline_at(args, geom, ranges, pinned, shape, prec, resident_vol):
                          staging=staging) as ctx:
         if args.slab:
             ctx.set_slab(*args.slab)
+        ctx.set_tile(shape.get("tile_y", 32))
         ctx.set_ranges(ranges)
         ctx.set_graphs(True)
         ctx.reconstruct_host(pinned, geom.matrices, ids)     # captures the batch graphs
@@ -434,6 +436,8 @@ def main():
     ap.add_argument("--views", type=int, default=496)
     ap.add_argument("--vpp", type=int, default=64, help="views per register pass (fused)")
     ap.add_argument("--run", type=int, default=16, help="voxels per thread along x")
+    ap.add_argument("--tile-y", type=int, default=32,
+                    help="staged tile y extent for 16-voxel runs (16, 32, 64) with --no-tune")
     ap.add_argument("--variants", default="plain-f32,fused-f32,fused-bf16,fused-f16")
     ap.add_argument("--headline", default="fused-f32")
     ap.add_argument("--cpu-views", type=int, default=0,
@@ -501,7 +505,8 @@ def main():
                         "shape": tuned["best"] if tuned else {"kernel": kernel,
                                                               "views_per_pass": vpp,
                                                               "run_length": args.run,
-                                                              "staging": True}}
+                                                              "staging": True,
+                                                              "tile_y": args.tile_y}}
         lat[var] = (ms_launch, launches, eff, vpp, prec)
         vols[var] = vol
         log(f"{var}: {ms:.2f} ms/step  {results[var]['gups']} GUPS "

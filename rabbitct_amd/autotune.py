@@ -1,9 +1,11 @@
 """Launch-shape autotuner for LolaHIP.
 
 Times candidate shapes (voxels per thread along x, views per register pass,
-LDS staging) on the views already resident in a context, HIP-event timed on
-the context's stream, and leaves the context set to the fastest.  The volume
-is restored afterwards (tuning launches accumulate into it)."""
+LDS staging, and for 16-voxel runs the staged tile's y extent: 16 x 32 x 8,
+16 x 16 x 16 or 16 x 64 x 4 voxels) on the views already resident in a
+context, HIP-event timed on the context's stream, and leaves the context set
+to the fastest.  The volume is restored afterwards (tuning launches
+accumulate into it)."""
 from __future__ import annotations
 
 import itertools
@@ -14,11 +16,22 @@ from . import lolahip
 
 RUNS = (8, 16, 32)
 PASSES = (8, 16, 32, 64)
+TILES_Y = (16, 32, 64)
 
 
 def candidates(precision: str):
-    stagings = (True, False)
-    return [("fused", vpp, run, st) for st, run, vpp in itertools.product(stagings, RUNS, PASSES)]
+    """(kernel, views per pass, run length, staging, tile y) tuples."""
+    out = []
+    for st, run, vpp in itertools.product((True, False), RUNS, PASSES):
+        for ty in (TILES_Y if run == 16 and st else (32,)):
+            out.append(("fused", vpp, run, st, ty))
+    return out
+
+
+def _apply(ctx, shape):
+    kern, vpp, run, st = shape[:4]
+    ctx.set_launch(kern, vpp, run, st)
+    ctx.set_tile(shape[4] if len(shape) > 4 else 32)
 
 
 def autotune(ctx: "lolahip.Context", count: int, precision: str = "f32", reps: int = 1,
@@ -28,20 +41,20 @@ def autotune(ctx: "lolahip.Context", count: int, precision: str = "f32", reps: i
     saved = ctx.get_volume()
     timings = {}
     for shape in shapes or candidates(precision):
-        kern, vpp, run, st = shape
         try:
-            ctx.set_launch(kern, vpp, run, st)
+            _apply(ctx, shape)
         except lolahip.LolaHipError:
             continue
         ctx.timed_backproject(count, 1)                      # warm the shape
         ms, _, _ = ctx.timed_backproject(count, reps)
         timings[shape] = ms
     best = min(timings, key=timings.get)
-    ctx.set_launch(*best)
+    _apply(ctx, best)
     ctx.set_volume(saved)
+    ty = lambda k: k[4] if len(k) > 4 else 32  # noqa: E731
     return {"best": {"kernel": best[0], "views_per_pass": best[1], "run_length": best[2],
-                     "staging": bool(best[3])},
-            "ms": {f"{k[0]} vpp={k[1]} R={k[2]} st={int(k[3])}": round(v, 3)
+                     "staging": bool(best[3]), "tile_y": ty(best)},
+            "ms": {f"{k[0]} vpp={k[1]} R={k[2]} st={int(k[3])} ty={ty(k)}": round(v, 3)
                    for k, v in timings.items()}}
 
 

@@ -250,17 +250,17 @@ __device__ __forceinline__ float l2_update(__amdgpu_buffer_rsrc_t rs, uint32_t b
 }
 
 // ---- LDS-staged tile ------------------------------------------------------
-// Tile of R (x) x 32 (y) x 8 (z) voxels: lane l of wave w owns
-// (x0..x0+R-1, y0 + l%32, z0 + 2w + l/32).  Per (view, tile) box_kernel
-// projects the tile's 8 corners and keeps the bounding box of their detector
-// positions (plus a 1-pixel margin and the right/lower tap); the kernel copies
-// that footprint of the bordered projection into LDS with direct-to-LDS
-// buffer loads (16 bytes per lane, two 128-pixel rows per instruction, no
-// VGPR round trip).  The footprints are double-buffered: view v+1 streams in
-// while view v computes, one barrier per view.  Voxel coordinates are
-// clamped to the box, or, for boxes the detector edge does not clip, lie in
-// it by construction, so every LDS index is in bounds.  A footprint larger
-// than kFW x kFH falls back to L2 gathers for that view.
+// Tile of R (x) x LY (y) x LZ (z) voxels (the Shape below): lane l of wave w
+// owns (x0..x0+R-1, y0 + l%LY, z0 + w*64/LY + l/LY).  Per (view, tile)
+// box_kernel projects the tile's 8 corners and keeps the bounding box of
+// their detector positions (plus a 1-pixel margin and the right/lower tap);
+// the kernel copies that footprint of the bordered projection into LDS with
+// direct-to-LDS buffer loads (16 bytes per lane, no VGPR round trip).  The
+// footprints are double-buffered: view v+1 streams in while view v computes,
+// one barrier per view.  Voxel coordinates are clamped to the box, or, for
+// boxes the detector edge does not clip, lie in it by construction, so every
+// LDS index is in bounds.  A footprint larger than the window falls back to
+// L2 gathers for that view.
 // updates per scheduling group in the staged loop, by precision (measured
 // at L=512 with R=16: FP32 fastest with groups of 16, that is one fence at
 // the end of the run, 47.4 against 48.5 ms per step with groups of 2 or 4;
@@ -275,25 +275,52 @@ constexpr int kFence = PREC == LH_PREC_F32 ? LH_FENCE32 : 4;
 // loads at a time is fastest, with R=32 four)
 template <int R>
 constexpr int kFenceL2 = R == 32 ? 4 : 1;
-#ifndef LH_PITCH
-#define LH_PITCH 128
-#endif
-#ifndef LH_FH
-#define LH_FH 39
-#endif
 #ifndef LH_OCC16
 #define LH_OCC16 4
 #endif
-constexpr int kFW    = 128;      // widest footprint staged (pixels)
-constexpr int kPitch = LH_PITCH; // LDS row pitch (dwords) >= kFW; ds_read2 offsets stay <= 255
-constexpr int kFH    = LH_FH;    // LDS rows per buffer (2 buffers x kFH x kPitch x 4 B <= 40 KB:
-                                 // 4 workgroups per CU)
-static_assert(kPitch >= kFW && kPitch + 1 <= 255, "LDS pitch");
-constexpr int kPitchLog2 = kPitch == 128 ? 7 : (kPitch == 256 ? 8 : -1);
-static_assert(kPitchLog2 > 0, "LDS pitch must be a power of two");
-static_assert(2 * kFH * kPitch * 4 <= 40 * 1024, "footprint buffers exceed 40 KB");
-constexpr int kLYs   = 32;  // lanes along y in the staged tile
-constexpr int kTileZ = 8;   // z extent of the staged tile
+
+// Shape of a staged tile and of its LDS footprint window.  A workgroup of 4
+// waves covers R (x, a template parameter of its own) x LY (y) x LZ (z) voxels,
+// LZ = 256 / LY: lane l of wave w owns the x-run at y0 + l % LY,
+// z0 + w * (64 / LY) + l / LY.  Per view the tile's detector footprint is
+// staged into a window of FH rows of PITCH dwords, double-buffered; a
+// footprint up to PITCH x FH pixels takes the staged path, a larger one L2
+// gathers.  The copy moves 1 KB chunks of the window (64 lanes x 16 bytes,
+// direct-to-LDS), wave w chunks w, w + 4, ...; lane l's chunk dword
+// 256 c + 4 l falls at row (256 c + 4 l) / PITCH, column (256 c + 4 l) % PITCH
+// (PITCH is a multiple of 4, so a lane's 16 bytes never straddle a row).
+// A power-of-two pitch window may end in a partial chunk (copied by the
+// lanes inside it); other windows are staged in whole chunks only, so a
+// footprint fits when its rows lie within them (FH rows, at most ROWS).
+// OCC: workgroups per CU the register and LDS budget is sized for.
+template <int LY_, int PITCH_, int ROWS_, int OCC_>
+struct Shape {
+  static constexpr int LY = LY_, LZ = 256 / LY_, ZW = kWave / LY_; // z per wave
+  static constexpr bool POW2  = (PITCH_ & (PITCH_ - 1)) == 0;
+  static constexpr int PITCH = PITCH_, FW = PITCH_, OCC = OCC_;
+  static constexpr int BUF    = PITCH_ * ROWS_;                   // dwords per buffer
+  static constexpr int CHUNKS = (BUF + 255) / 256;                // 1 KB chunks per buffer
+  static constexpr int KMAX   = (CHUNKS + kWaves - 1) / kWaves;   // chunks per wave
+  static constexpr int FH     = POW2 ? ROWS_ : (BUF / 256) * 256 / PITCH_;
+  static constexpr int LOG2   = PITCH_ == 64 ? 6 : PITCH_ == 128 ? 7 : PITCH_ == 256 ? 8 : 0;
+  static_assert(PITCH_ % 4 == 0 && PITCH_ + 1 <= 255, "ds_read2 offsets are 8-bit dwords");
+  static_assert(!POW2 || LOG2 > 0, "power-of-two pitch without its shift");
+  static_assert(LY_ * (256 / LY_) == 256 && kWave % LY_ == 0, "lane layout");
+  static_assert(OCC_ * (2 * BUF * 4 + 2 * kWaves * 4) <= 160 * 1024, "LDS of OCC workgroups");
+  static_assert(FH >= 2, "window rows");
+};
+// 32 (y) x 8 (z) tiles, 128 x 39 window, 4 workgroups per CU: every footprint
+// of a 16-voxel run fits at L=512 (and of a 32-voxel run at L=1024)
+typedef Shape<32, 128, 39, LH_OCC16> ShapeY32;
+// 32 x 8 tiles of 32-voxel runs where they span 16 mm (L <= 512): a 160 x 40
+// window (at L=512 99.7% of the footprints fit 160 x 39, against 90.6% at
+// 128), three workgroups per CU
+typedef Shape<32, 160, 40, 3> ShapeY32W;
+// 16 x 16 tiles: narrow, tall footprints (80 x 60: at L=512 every 16-voxel
+// run's fits)
+typedef Shape<16, 80, 62, 4> ShapeY16;
+// 64 x 4 tiles: wide, short footprints (224 x 21)
+typedef Shape<64, 224, 22, 4> ShapeY64;
 
 struct StageBox {
   int iu0, iv0;   // footprint origin (pixels)
@@ -326,7 +353,7 @@ __device__ __forceinline__ uint2 fetch_box(__amdgpu_buffer_rsrc_t rs, Args &A, i
 
 // STAGE = 0: every footprint is gathered from L2 (the staged tile shape and
 // coordinate arithmetic without the LDS copies).
-template <int STAGE>
+template <int STAGE, class SH>
 __device__ __forceinline__ StageBox decode_box(uint2 raw)
 {
   const int qx = __builtin_amdgcn_readfirstlane((int)raw.x);
@@ -337,7 +364,7 @@ __device__ __forceinline__ StageBox decode_box(uint2 raw)
   b.fw       = qy & 0x7fff;
   b.interior = (qy & 0x8000) != 0;
   b.fh       = qy >> 16;
-  b.fits     = STAGE && b.fw >= 2 && b.fh >= 2 && b.fw <= kFW && b.fh <= kFH;
+  b.fits     = STAGE && b.fw >= 2 && b.fh >= 2 && b.fw <= SH::FW && b.fh <= SH::FH;
   return b;
 }
 
@@ -352,7 +379,8 @@ __device__ __forceinline__ StageBox decode_box(uint2 raw)
 // so the clamp is the identity for the whole tile and the coordinates are
 // positive (truncation is the floor).
 __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, int nTiles,
-    int tilesX, int tilesY, int R, float O_L, float R_L, int W, int H, int nSlots, int2 *boxes)
+    int tilesX, int tilesY, int R, int ly, int lz, float O_L, float R_L, int W, int H,
+    int nSlots, int2 *boxes)
 {
   const int t    = blockIdx.x * blockDim.x + threadIdx.x;
   const int slot = firstSlot + blockIdx.y;
@@ -365,8 +393,8 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
   bool interior = true;
   for (int c = 0; c < 8; ++c) {
     const double x = O + (double)(tx * R + ((c & 1) ? R - 1 : 0)) * RL;
-    const double y = O + (double)(ty * kLYs + ((c & 2) ? kLYs - 1 : 0)) * RL;
-    const double z = O + (double)(tz * kTileZ + ((c & 4) ? kTileZ - 1 : 0)) * RL;
+    const double y = O + (double)(ty * ly + ((c & 2) ? ly - 1 : 0)) * RL;
+    const double z = O + (double)(tz * lz + ((c & 4) ? lz - 1 : 0)) * RL;
     const double hw = fma(a[2], x, fma(a[5], y, fma(a[8], z, a[11])));
     const double u  = fma(a[0], x, fma(a[3], y, fma(a[6], z, a[9]))) / hw;
     const double v  = fma(a[1], x, fma(a[4], y, fma(a[7], z, a[10]))) / hw;
@@ -388,52 +416,96 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
   boxes[(uint64_t)t * (uint64_t)nSlots + slot] = q;
 }
 
-// Issues the direct-to-LDS loads of footprint `b` of the slot image into buf.
-// FP32: each buffer_load_dwordx4 ... lds moves two rows of 128 pixels (lane l
-// lands 16 bytes at l*16, so lanes 32-63 fill the next LDS row); wave w copies
-// row pairs 2w, 2w+8, ... (a half image holds one dword per element too:
-// p(i,j) and the forward difference p(i+1,j) - p(i,j)).  laneOff is the lane's
-// byte offset within a row pair, fixed per tile; the loop control is
-// wave-uniform and only the lower half-wave copies a last odd row.  Other
-// layouts: one dword per lane,
-// pixels [iu0 + c0, iu0 + c0 + 64) of a row per instruction, wave w copying
-// rows w, w+4, ....  Columns at or beyond fw are copied too and never
-// read; past the end of the slot the buffer bounds return zero.  Completion is
-// awaited by the caller (s_waitcnt vmcnt(0) and a barrier).
-template <int ELEM>
-__device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StageBox &b,
-    int pitch, float *buf, int wv, uint32_t laneOff, bool loHalf)
-{
-  const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * ELEM;
-  if constexpr (ELEM == 4 && kPitch == 128) {
-    uint32_t off        = base + laneOff + (uint32_t)(2 * wv * pitch) * 4u;
-    const uint32_t step = (uint32_t)(2 * kWaves * pitch) * 4u;
-    int r0              = 2 * wv;
-    for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += step)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-          (__attribute__((address_space(3))) void *)(buf + r0 * kPitch), 16, (int)off, 0, 0, 0);
-    if (r0 < b.fh && loHalf)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-          (__attribute__((address_space(3))) void *)(buf + r0 * kPitch), 16, (int)off, 0, 0, 0);
-  } else {
-    const int lane = threadIdx.x & (kWave - 1);
-    for (int r = wv; r < b.fh; r += kWaves) {
-      const uint32_t row = base + (uint32_t)(r * pitch) * ELEM;
-      for (int c0 = 0; c0 < b.fw; c0 += kWave) {
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-            (__attribute__((address_space(3))) void *)(buf + r * kPitch + c0), 4,
-            (int)(row + (uint32_t)(c0 + lane) * ELEM), 0, 0, 0);
+// Issues the direct-to-LDS loads of footprint `b` of the slot image into buf
+// (one dword per element: an FP32 pixel, or for half storage p(i,j) and the
+// forward difference p(i+1,j) - p(i,j)).  The window is copied in 1 KB
+// chunks, each one buffer_load_dwordx4 ... lds (lane l lands 16 bytes at
+// 16 l); wave w copies chunks w, w + 4, ... of those that hold rows < fh.
+// rel(k) is the lane's source byte offset, relative to the box origin, in its
+// k-th chunk: for a 128-dword pitch the chunks are row pairs and rel(k) is
+// linear in k; otherwise a per-lane table made once per tile.  Columns at or
+// beyond fw and rows at or beyond fh within a copied chunk are copied too and
+// never read; no lane writes past the end of the window; past the end of the
+// slot the buffer bounds return zero.  Completion is awaited by the caller
+// (s_waitcnt vmcnt(0) and a barrier).
+template <class SH>
+struct ChunkOffsets {
+  static_assert(!SH::POW2 || 1024 % SH::PITCH == 0, "whole rows per 4 chunks");
+  uint32_t rel[SH::POW2 ? 1 : SH::KMAX];
+  uint32_t lane0; // 128-dword pitch: rel(0)
+  uint32_t step;  // 128-dword pitch: rel(k+1) - rel(k)
+  __device__ __forceinline__ void init(int wv, int lane, uint32_t rowBytes)
+  {
+    if constexpr (SH::POW2) {
+      const uint32_t d = (uint32_t)(256 * wv + 4 * lane);
+      lane0 = (d >> SH::LOG2) * rowBytes + (d & (SH::PITCH - 1)) * 4u;
+      step  = (uint32_t)(1024 / SH::PITCH) * rowBytes;
+    } else {
+#pragma unroll
+      for (int k = 0; k < SH::KMAX; ++k) {
+        const uint32_t d = (uint32_t)(256 * (wv + kWaves * k) + 4 * lane);
+        rel[k] = (d / SH::PITCH) * rowBytes + (d % SH::PITCH) * 4u;
       }
+    }
+  }
+  __device__ __forceinline__ uint32_t at(int k) const
+  {
+    if constexpr (SH::POW2)
+      return lane0 + (uint32_t)k * step;
+    else
+      return rel[k];
+  }
+};
+
+template <class SH>
+__device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StageBox &b,
+    int pitch, float *buf, int wv, int lane, const ChunkOffsets<SH> &co)
+{
+  const uint32_t base = (uint32_t)((b.iv0 + (int)kBorder) * pitch + b.iu0 + (int)kBorder) * 4u;
+  const int need      = (b.fh * SH::PITCH + 255) >> 8; // chunks holding rows < fh
+  constexpr int kTail = SH::BUF & 255;                   // dwords of a partial last chunk
+  constexpr int kWhole = kTail ? SH::CHUNKS - 1 : SH::CHUNKS;
+  if constexpr (SH::POW2) {
+    // 128-dword pitch: a chunk is a row pair, wave w copies pairs w, w + 4,
+    // ... in a rolled loop, and the lower half-wave alone a last odd row (the
+    // window's last row too when FH is odd).  Measured 2.5% faster per step
+    // than a loop over chunk counts at L=512.
+    static_assert(SH::PITCH == 128, "row pairs");
+    uint32_t off = base + co.lane0;
+    int r0       = 2 * wv;
+    for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += co.step)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0, 0,
+          0);
+    if (r0 < b.fh && lane < 32)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0, 0,
+          0);
+  } else {
+    // whole chunks only (FH keeps need <= CHUNKS); the opaque copy of the
+    // wave index keeps the per-chunk bounds and LDS offsets from being
+    // hoisted out of the view loop into registers of their own
+    static_assert(kTail == 0 || SH::FH * SH::PITCH <= 256 * kWhole, "tail chunk");
+    int w = wv;
+    __asm__("" : "+s"(w));
+#pragma unroll
+    for (int k = 0; k < SH::KMAX; ++k) {
+      const int ch = w + kWaves * k;
+      if (ch >= need)
+        break;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + 256 * ch), 16,
+          (int)(base + co.at(k)), 0, 0, 0);
     }
   }
 }
 
 // One voxel-view update from the LDS footprint (LolaOPT.c:145-170) at local
-// homogeneous (u', v', w): returns val / w^2.  sBase[v * kPitch + u] is
+// homogeneous (u', v', w): returns val / w^2.  sBase[v * PITCH + u] is
 // footprint pixel (u, v) (kEdge: sBase is offset by the box origin, so it is
 // indexed by global pixels).  The coordinate arithmetic is cell_of's, shared
 // with l2_update.
-template <int PREC, int MODE>
+template <int PREC, int MODE, class SH>
 __device__ __forceinline__ float lds_update(const float *sBase, float u, float v, float w,
     float hiu, float hiv, float ou, float ov)
 {
@@ -442,33 +514,38 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
   cell_of<MODE>(u * rw, v * rw, hiu, hiv, ou, ov, ax, ay, fx, fy);
   const float *q;
   if constexpr (MODE == kInterior) {
-    // byte address base + 4 * column + row * 4 * kPitch as two shift-adds: the
-    // empty asm hides the partial sum, which the compiler would otherwise
-    // re-associate into two shifts and a three-way add
+    // byte address base + 4 * column + row * 4 * PITCH as a shift-add and a
+    // shift-add (power-of-two pitch) or a 24-bit multiply-add: the empty asm
+    // hides the partial sum, which the compiler would otherwise re-associate
+    // into two shifts and a three-way add
     typedef const __attribute__((address_space(3))) float LdsFloat;
     uint32_t col = (uint32_t)(uintptr_t)(LdsFloat *)sBase + ((uint32_t)(int)fx << 2);
     __asm__("" : "+v"(col));
-    q = (const float *)(LdsFloat *)(uintptr_t)(col + ((uint32_t)(int)fy << (kPitchLog2 + 2)));
+    if constexpr (SH::POW2)
+      q = (const float *)(LdsFloat *)(uintptr_t)(col + ((uint32_t)(int)fy << (SH::LOG2 + 2)));
+    else
+      q = (const float *)(LdsFloat *)(uintptr_t)(
+          __umul24((uint32_t)(int)fy, 4u * SH::PITCH) + col);
   } else {
-    // fy*kPitch + fx is an exact integer in FP32 (|.| < 2^24): one conversion
-    q = sBase + (int)fmaf(fy, (float)kPitch, fx);
+    // fy*PITCH + fx is an exact integer in FP32 (|.| < 2^24): one conversion
+    q = sBase + (int)fmaf(fy, (float)SH::PITCH, fx);
   }
   float b, t;
   if constexpr (PREC == LH_PREC_F32) {
     const float p00 = q[0], p10 = q[1];
-    const float p01 = q[kPitch], p11 = q[kPitch + 1];
+    const float p01 = q[SH::PITCH], p11 = q[SH::PITCH + 1];
     b = fmaf(ax, p10 - p00, p00);
     t = fmaf(ax, p11 - p01, p01);
   } else {
     // one dword per pixel: p(i,j) and the forward difference to p(i+1,j)
     const uint32_t *qq = reinterpret_cast<const uint32_t *>(q);
     b = lerp_pair<PREC>(qq[0], ax);
-    t = lerp_pair<PREC>(qq[kPitch], ax);
+    t = lerp_pair<PREC>(qq[SH::PITCH], ax);
   }
   return fmaf(ay, t - b, b) * (rw * rw);
 }
 
-template <int R, int PREC, int STAGE>
+template <int R, int PREC, int STAGE, class SH>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
 {
@@ -479,10 +556,10 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   const int lane = threadIdx.x & (kWave - 1);
   const int wv   = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int x0   = tx * R;
-  const int y0   = ty * kLYs;
-  const int z0   = tz * kTileZ;
-  const int y    = y0 + (lane & (kLYs - 1));
-  const int z    = z0 + 2 * wv + lane / kLYs;
+  const int y0   = ty * SH::LY;
+  const int z0   = tz * SH::LZ;
+  const int y    = y0 + (lane & (SH::LY - 1));
+  const int z    = z0 + SH::ZW * wv + lane / SH::LY;
   const int L    = A.L;
   const double O = A.O_L, RL = A.R_L;
 
@@ -493,9 +570,9 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   const uint64_t LL   = (uint64_t)L * (uint64_t)L;
   const int pitch     = (int)(A.rowBytes / ELEM);
   const int n         = A.nViews;
-  // the lane's byte offset within a staged row pair (16 bytes per lane)
-  const uint32_t laneOff = (uint32_t)(((lane >> 5) * pitch + (lane & 31) * 4) * ELEM);
-  const bool loHalf      = lane < 32;
+  // the lane's source offsets in the footprint copies
+  ChunkOffsets<SH> co;
+  co.init(wv, lane, A.rowBytes);
 
   auto range_of = [&](int v) -> int {
     return A.clip ? A.ranges[(uint64_t)(view_params(A, A.firstSlot + v).id - 1) * LL + line]
@@ -525,11 +602,11 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   __syncthreads();
   bool blockAct = __builtin_amdgcn_readfirstlane(sAct[0] | sAct[1] | sAct[2] | sAct[3]) != 0;
   const __amdgpu_buffer_rsrc_t boxRs = box_rsrc(A);
-  StageBox box = decode_box<STAGE>(fetch_box(boxRs, A, A.firstSlot, tile));
+  StageBox box = decode_box<STAGE, SH>(fetch_box(boxRs, A, A.firstSlot, tile));
   // box of view v+1, in flight during view v-1
   uint2 boxRaw = n > 1 ? fetch_box(boxRs, A, A.firstSlot + 1, tile) : make_uint2(0u, 0u);
   if (blockAct && box.fits)
-    stage_issue<ELEM>(slot_rsrc(A, A.firstSlot), box, pitch, sImg, wv, laneOff, loHalf);
+    stage_issue<SH>(slot_rsrc(A, A.firstSlot), box, pitch, sImg, wv, lane, co);
 
   for (int v = 0; v < n; ++v) {
     const int slot      = A.firstSlot + v;
@@ -549,13 +626,13 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       rr_nxt = range_of(v + 2);
     StageBox boxNext = box;
     if (hasNext) {
-      boxNext = decode_box<STAGE>(boxRaw);
+      boxNext = decode_box<STAGE, SH>(boxRaw);
       KernelArgs &K = kernel_args();
       if (v + 2 < n)
         boxRaw = fetch_box(box_rsrc(K), K, slot + 2, tile);
       if (blockActNext && boxNext.fits)
-        stage_issue<ELEM>(slot_rsrc(K, slot + 1), boxNext, pitch,
-            sImg + ((v + 1) & 1) * kPitch * kFH, wv, laneOff, loHalf);
+        stage_issue<SH>(slot_rsrc(K, slot + 1), boxNext, pitch,
+            sImg + ((v + 1) & 1) * SH::BUF, wv, lane, co);
     }
 
     const int rlo     = lo_c;
@@ -608,7 +685,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       const bool edge = box.iu0 < 0 || box.iv0 < 0;
       if (box.fits) {
         // sImg buffer of this view
-        const float *sBuf = sImg + (v & 1) * kPitch * kFH;
+        const float *sBuf = sImg + (v & 1) * SH::BUF;
         // groups of kFence updates are fenced so the scheduler keeps at most
         // that many updates' taps in flight per lane
         if (full && box.interior) {
@@ -617,7 +694,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
-            acc[r] += lds_update<PREC, kInterior>(sBuf, r ? fmaf(fr, du, uo) : uo,
+            acc[r] += lds_update<PREC, kInterior, SH>(sBuf, r ? fmaf(fr, du, uo) : uo,
                 r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
             if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
@@ -627,7 +704,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
-            acc[r] += lds_update<PREC, kClamp>(sBuf, r ? fmaf(fr, du, uo) : uo,
+            acc[r] += lds_update<PREC, kClamp, SH>(sBuf, r ? fmaf(fr, du, uo) : uo,
                 r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
             if ((r % kFence<PREC>) == kFence<PREC> - 1)
               __builtin_amdgcn_sched_barrier(0);
@@ -641,12 +718,12 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
           // and x - trunc(x) is fract(x))
           const float ou = edge ? (float)box.iu0 : 0.0f, ov = edge ? (float)box.iv0 : 0.0f;
           // indexed by global pixels for edge boxes
-          const float *sB = edge ? sBuf - (box.iv0 * kPitch + box.iu0) : sBuf;
+          const float *sB = edge ? sBuf - (box.iv0 * SH::PITCH + box.iu0) : sBuf;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float fr = (float)r;
             const bool on  = r >= rlo && r < rhi;
-            acc[r] += lds_update<PREC, kEdge>(sB, on ? fmaf(fr, du, uo) : kParkUV,
+            acc[r] += lds_update<PREC, kEdge, SH>(sB, on ? fmaf(fr, du, uo) : kParkUV,
                 on ? fmaf(fr, dv, vo) : kParkUV, on ? fmaf(fr, dw, wo) : kPark, hiu, hiv, ou,
                 ov);
             if ((r % kFence<PREC>) == kFence<PREC> - 1)
@@ -703,18 +780,15 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 // its L2 serves one band of detector rows.
 // A must stay the kernel's only argument: kernel_args() reads it back from
 // offset 0 of the kernarg segment.
-// OCC: workgroups per CU the register budget is sized for (4: 128 VGPRs).
-// 32-voxel runs hold 32 accumulators; with OCC = 2 they may use more
-// registers, which their L2-gather fallback needs to keep four voxels' taps in
-// flight; the launcher picks OCC = 4 for them where footprints fit the LDS
-// window (small tiles).
-template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : LH_OCC16)>
-__global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
+// SH::OCC: workgroups per CU the register and LDS budget is sized for
+// (4: 128 VGPRs, 3: 168).
+template <int R, int PREC, int STAGE, class SH>
+__global__ __launch_bounds__(kWaves *kWave, SH::OCC) void bp_kernel(BpArgs A)
 {
-  __shared__ float sImg[STAGE ? 2 * kPitch * kFH : 1];
+  __shared__ float sImg[STAGE ? 2 * SH::BUF : 1];
   __shared__ int sAct[2 * kWaves];
   if (!A.persistent) {
-    bp_tile_lds<R, PREC, STAGE>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z, sImg, sAct);
+    bp_tile_lds<R, PREC, STAGE, SH>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z, sImg, sAct);
     return;
   }
   const int nTiles  = A.tilesX * A.tilesY * A.tilesZ;
@@ -730,7 +804,7 @@ __global__ __launch_bounds__(kWaves *kWave, OCC) void bp_kernel(BpArgs A)
     const int ty = (t / A.tilesX) % A.tilesY;
     const int tz = A.tileZ0 + t / (A.tilesX * A.tilesY);
     __syncthreads(); // the previous tile's last LDS reads (footprints, votes) are done
-    bp_tile_lds<R, PREC, STAGE>(A, tx, ty, tz, sImg, sAct);
+    bp_tile_lds<R, PREC, STAGE, SH>(A, tx, ty, tz, sImg, sAct);
   }
 }
 
@@ -1023,26 +1097,37 @@ struct LhContext {
   int2 *dBoxes  = nullptr;          // staged kernel: footprint box per (slot, tile)
   int2 *scratchBoxes = nullptr;     // device address of `scratch` on this context's device
   long boxTiles = 0;                // tiles per slot in dBoxes
-  std::vector<int> boxR;            // run length the slot's boxes were built for (0 = none)
+  std::vector<int> boxR;            // tile (run length, y extent) the slot's boxes were built
+                                    // for, as R | ly << 8 (0 = none)
+  uint32_t tileY = 0;               // tile y extent for 16-voxel runs (0 = 32)
 };
 
 // Grid of one launch: one workgroup per tile, or for the persistent form as
 // many workgroups as the chip keeps resident (occupancy x CUs).
-template <int R, int PREC, int STAGE, int OCC = (R > 16 ? 2 : LH_OCC16)>
+template <int R, int PREC, int STAGE, class SH>
 static void launch_shape(LhContext *c, const BpArgs &A)
 {
   static int perCU = 0;
   dim3 grid(A.tilesX, A.tilesY, A.tilesZ);
   if (A.persistent) {
     if (perCU == 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU,
-                           bp_kernel<R, PREC, STAGE, OCC>, kWaves * kWave, 0) != hipSuccess ||
+                           bp_kernel<R, PREC, STAGE, SH>, kWaves * kWave, 0) != hipSuccess ||
                           perCU < 1))
-      perCU = 4;
+      perCU = SH::OCC;
     grid = dim3((unsigned)std::min<long>((long)c->numCU * perCU,
         (long)A.tilesX * A.tilesY * A.tilesZ));
   }
-  bp_kernel<R, PREC, STAGE, OCC><<<grid, dim3(kWaves * kWave), 0, c->stream>>>(A);
+  bp_kernel<R, PREC, STAGE, SH><<<grid, dim3(kWaves * kWave), 0, c->stream>>>(A);
 }
+
+// Staged tile shape of a launch: (y, z) extents of the tile and the window
+// kernel instance it runs.  The context's tile choice (lhSetTile) picks the
+// y extent for 16-voxel runs; 8- and 32-voxel runs take 32 x 8 tiles.
+enum TileKind { kTileY32, kTileY32W, kTileY16, kTileY64 };
+struct TileDims {
+  int ly, lz, kind;
+};
+static TileDims tile_dims(const LhContext *c);
 
 const LhConfig &lh_config(const LhContext *c) { return c->cfg; }
 hipStream_t lh_stream(const LhContext *c) { return c->stream; }
@@ -1529,6 +1614,21 @@ int lhHostFree(void *p)
   return LH_OK;
 }
 
+static TileDims tile_dims(const LhContext *c)
+{
+  const LhConfig &g = c->cfg;
+  if (g.runLength == 16 && c->tileY == 16)
+    return {16, 16, kTileY16};
+  if (g.runLength == 16 && c->tileY == 64)
+    return {64, 4, kTileY64};
+  // A 32-voxel run spanning more than 8 mm along x (L <= 512) has footprints
+  // wider than 128 pixels at some views: the 160-pixel window, 3 workgroups
+  // per CU.  At 8 mm and below (L >= 1024) the 128-pixel window fits them at 4.
+  if (g.runLength == 32 && 32.0f * g.R_L > 8.0f)
+    return {32, 8, kTileY32W};
+  return {32, 8, kTileY32};
+}
+
 static void drop_graphs(LhContext *c)
 {
   for (auto &kv : c->graphCache)
@@ -1560,29 +1660,30 @@ static int box_storage(LhContext *c, long nTiles)
 // Footprint boxes of slots [first, first+count) for the staged tile grid of
 // run length R; built once per (slot, R) and kept until the slot is reloaded
 // (with `force`, rebuilt for every slot: the form a captured batch replays).
-static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, int tilesX,
-    int tilesY, int tilesZ, bool force)
+static int ensure_boxes(LhContext *c, uint32_t first, uint32_t count, int R, const TileDims &td,
+    int tilesX, int tilesY, int tilesZ, bool force)
 {
   const LhConfig &g = c->cfg;
   const long nTiles = (long)tilesX * tilesY * tilesZ;
+  const int key     = R | td.ly << 8;
   const int rc      = box_storage(c, nTiles);
   if (rc != LH_OK)
     return rc;
   uint32_t s = first;
   while (s < first + count) {
-    if (!force && c->boxR[s] == R) {
+    if (!force && c->boxR[s] == key) {
       ++s;
       continue;
     }
     uint32_t e = s;
-    while (e < first + count && (force || c->boxR[e] != R) && e - s < 65535)
+    while (e < first + count && (force || c->boxR[e] != key) && e - s < 65535)
       ++e;
     box_kernel<<<dim3((unsigned)((nTiles + 255) / 256), e - s), 256, 0, c->stream>>>(c->dViews,
-        (int)s, (int)nTiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H,
+        (int)s, (int)nTiles, tilesX, tilesY, R, td.ly, td.lz, g.O_L, g.R_L, (int)g.W, (int)g.H,
         (int)g.nSlots, c->dBoxes);
     LH_HIP(hipGetLastError());
     for (uint32_t k = s; k < e; ++k)
-      c->boxR[k] = R;
+      c->boxR[k] = key;
     s = e;
   }
   return LH_OK;
@@ -1615,15 +1716,19 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   A.clip        = g.clip;
   const int R   = (int)g.runLength;
   const int stage = g.staging ? 1 : 0;
+  const TileDims td = tile_dims(c);
+  if (c->slabZ0 % td.lz || c->slabZ1 % td.lz)
+    return fail(LH_E_ARG, "slab [%u, %u) is not whole %d-voxel tiles along z", c->slabZ0,
+        c->slabZ1, td.lz);
   // both tap paths walk the same tiles and footprint boxes (the boxes fix
   // the local coordinate origin)
   A.tilesX      = (int)g.L / R;
-  A.tilesY      = (int)g.L / kLYs;
-  A.tilesZ      = (int)g.L / kTileZ;
+  A.tilesY      = (int)g.L / td.ly;
+  A.tilesZ      = (int)g.L / td.lz;
   // the box table covers every z tile; a slab launch walks its own
   const int tilesZAll = A.tilesZ;
-  A.tileZ0      = (int)c->slabZ0 / kTileZ;
-  A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / kTileZ;
+  A.tileZ0      = (int)c->slabZ0 / td.lz;
+  A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / td.lz;
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
   A.boxSlots    = (int)g.nSlots;
   if (extBoxes) {
@@ -1634,48 +1739,38 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
     A.boxSlots  = (int)count;
     A.boxBytes  = (uint32_t)((uint64_t)A.tilesX * A.tilesY * tilesZAll * count * sizeof(int2));
   } else {
-    const int rc = ensure_boxes(c, first, count, R, A.tilesX, A.tilesY, tilesZAll, forceBoxes);
+    const int rc =
+        ensure_boxes(c, first, count, R, td, A.tilesX, A.tilesY, tilesZAll, forceBoxes);
     if (rc != LH_OK)
       return rc;
     A.boxes    = c->dBoxes;
     A.boxBytes = (uint32_t)((uint64_t)g.nSlots * (uint64_t)c->boxTiles * sizeof(int2));
   }
-#define LH_LAUNCH(RR, PP, SS) launch_shape<RR, PP, SS>(c, A)
-#define LH_BY_PREC(RR)                                                                   \
+#define LH_BY_PREC(RR, SH)                                                               \
   switch (g.precision) {                                                                 \
   case LH_PREC_F32:                                                                      \
-    if (stage) LH_LAUNCH(RR, LH_PREC_F32, 1); else LH_LAUNCH(RR, LH_PREC_F32, 0);          \
+    if (stage) launch_shape<RR, LH_PREC_F32, 1, SH>(c, A);                                \
+    else launch_shape<RR, LH_PREC_F32, 0, SH>(c, A);                                      \
     break;                                                                               \
   case LH_PREC_BF16:                                                                     \
-    if (stage) LH_LAUNCH(RR, LH_PREC_BF16, 1); else LH_LAUNCH(RR, LH_PREC_BF16, 0);        \
+    if (stage) launch_shape<RR, LH_PREC_BF16, 1, SH>(c, A);                               \
+    else launch_shape<RR, LH_PREC_BF16, 0, SH>(c, A);                                     \
     break;                                                                               \
   default:                                                                               \
-    if (stage) LH_LAUNCH(RR, LH_PREC_F16, 1); else LH_LAUNCH(RR, LH_PREC_F16, 0);          \
+    if (stage) launch_shape<RR, LH_PREC_F16, 1, SH>(c, A);                                \
+    else launch_shape<RR, LH_PREC_F16, 0, SH>(c, A);                                      \
     break;                                                                               \
   }
-  // A staged 32-voxel run whose tile spans at most 8 mm along x (L=1024 and
-  // beyond) has footprints that fit the LDS window like the 16-voxel tiles
-  // at L=512: it runs at 4 workgroups per CU (128 VGPRs), 11% faster at
-  // L=1024.  Larger tiles (L <= 512) take the L2 fallback often, which is
-  // 1.5x slower within 128 VGPRs, so they keep the larger register budget.
-  const bool compact32 = stage && 32.0f * g.R_L <= 8.0f;
-  switch (R) {
-  case 8: LH_BY_PREC(8); break;
-  case 32:
-    if (compact32) {
-      switch (g.precision) {
-      case LH_PREC_F32: launch_shape<32, LH_PREC_F32, 1, 4>(c, A); break;
-      case LH_PREC_BF16: launch_shape<32, LH_PREC_BF16, 1, 4>(c, A); break;
-      default: launch_shape<32, LH_PREC_F16, 1, 4>(c, A); break;
-      }
-    } else {
-      LH_BY_PREC(32);
-    }
-    break;
-  default: LH_BY_PREC(16); break;
+  switch (R * 16 + td.kind) {
+  case 8 * 16 + kTileY32: LH_BY_PREC(8, ShapeY32); break;
+  case 16 * 16 + kTileY32: LH_BY_PREC(16, ShapeY32); break;
+  case 16 * 16 + kTileY16: LH_BY_PREC(16, ShapeY16); break;
+  case 16 * 16 + kTileY64: LH_BY_PREC(16, ShapeY64); break;
+  case 32 * 16 + kTileY32: LH_BY_PREC(32, ShapeY32); break;
+  case 32 * 16 + kTileY32W: LH_BY_PREC(32, ShapeY32W); break;
+  default: return fail(LH_E_ARG, "no %d x %d x %d staged tile", R, td.ly, td.lz);
   }
 #undef LH_BY_PREC
-#undef LH_LAUNCH
   LH_HIP(hipGetLastError());
   return LH_OK;
 }
@@ -1687,14 +1782,16 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
 static int graph_backproject(LhContext *c, uint32_t first, uint32_t count, uint32_t per)
 {
   const LhConfig &g = c->cfg;
-  const auto key = std::make_tuple(first, count, g.kernel * 4 + g.precision, (int)g.runLength,
+  const TileDims td = tile_dims(c);
+  const auto key = std::make_tuple(first, count, g.kernel * 4 + g.precision,
+      (int)g.runLength | td.ly << 8,
       g.kernel == LH_KERNEL_FUSED ? (int)(per << 1 | (uint32_t)g.staging) : g.staging);
   auto it = c->graphCache.find(key);
   if (it == c->graphCache.end()) {
     {
       // allocate the box table outside the capture
       const int R  = (int)g.runLength;
-      const int rc = box_storage(c, (long)(g.L / R) * (g.L / kLYs) * (g.L / kTileZ));
+      const int rc = box_storage(c, (long)(g.L / R) * (g.L / td.ly) * (g.L / td.lz));
       if (rc != LH_OK)
         return rc;
     }
@@ -1718,7 +1815,7 @@ static int graph_backproject(LhContext *c, uint32_t first, uint32_t count, uint3
   }
   LH_HIP(hipGraphLaunch(it->second, c->stream));
   for (uint32_t s = first; s < first + count; ++s)
-    c->boxR[s] = (int)g.runLength;
+    c->boxR[s] = (int)g.runLength | td.ly << 8;
   return LH_OK;
 }
 
@@ -2035,7 +2132,8 @@ int lhBackprojectNoAlloc(LhContext *c, uint32_t first, uint32_t count)
     return fail(LH_E_STATE, "lhBackprojectNoAlloc: scratch table not resolved");
   const uint32_t per   = g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 1u;
   const int R          = (int)g.runLength;
-  const int tilesX     = (int)g.L / R, tilesY = (int)g.L / kLYs, tilesZ = (int)g.L / kTileZ;
+  const TileDims td    = tile_dims(c);
+  const int tilesX     = (int)g.L / R, tilesY = (int)g.L / td.ly, tilesZ = (int)g.L / td.lz;
   const uint64_t tiles = (uint64_t)tilesX * tilesY * tilesZ;
   const uint64_t need  = tiles * std::min(per, count);
   if (need > kScratchBoxes)
@@ -2050,7 +2148,7 @@ int lhBackprojectNoAlloc(LhContext *c, uint32_t first, uint32_t count)
   for (uint32_t s = first; s < first + count; s += per) {
     const uint32_t n = std::min(per, first + count - s);
     box_kernel<<<dim3((unsigned)((tiles + 255) / 256), n), 256, 0, c->stream>>>(c->dViews + s, 0,
-        (int)tiles, tilesX, tilesY, R, g.O_L, g.R_L, (int)g.W, (int)g.H, (int)n,
+        (int)tiles, tilesX, tilesY, R, td.ly, td.lz, g.O_L, g.R_L, (int)g.W, (int)g.H, (int)n,
         c->scratchBoxes);
     LH_HIP(hipGetLastError());
     const int rc = launch_bp(c, s, n, false, c->scratchBoxes);
@@ -2124,6 +2222,16 @@ int lhSetLaunch(LhContext *c, int kernel, uint32_t viewsPerPass, uint32_t runLen
   c->cfg.viewsPerPass = viewsPerPass;
   c->cfg.runLength    = runLength;
   c->cfg.staging      = staging ? 1 : 0;
+  return LH_OK;
+}
+
+int lhSetTile(LhContext *c, uint32_t tileY)
+{
+  if (!c || (tileY != 0 && tileY != 16 && tileY != 32 && tileY != 64))
+    return fail(LH_E_ARG, "lhSetTile: tile y extent must be 16, 32 or 64 (0: 32)");
+  if (tileY == 16 && (c->slabZ0 % 16 || c->slabZ1 % 16))
+    return fail(LH_E_ARG, "lhSetTile: 16 x 16 tiles need a slab of multiples of 16");
+  c->tileY = tileY == 32 ? 0 : tileY;
   return LH_OK;
 }
 

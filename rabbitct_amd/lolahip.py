@@ -75,7 +75,7 @@ EXPORTS = (
     "lhSynchronize", "lhEffectiveUpdates", "lhLastError", "lhAbiLayout", "lhGenerateViews",
     "lhSetLaunch", "lhCopyCalibration", "lhVolumeDevice", "lhScaleVolume",
     "lhAcquireUpload", "lhCommitUpload", "lhCancelUpload", "lhSetGraphs", "lhSetSlab",
-    "lhGatherRoof",
+    "lhGatherRoof", "lhSetTile",
     # sparse-view + learned correction
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
@@ -147,6 +147,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhCancelUpload.argtypes = [P]
     lib.lhSetGraphs.argtypes = [P, i32]
     lib.lhSetSlab.argtypes = [P, u32, u32]
+    lib.lhSetTile.argtypes = [P, u32]
     lib.lhGatherRoof.argtypes = [i32, i32, i32, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(f32)]
     u64, f64 = ctypes.c_uint64, ctypes.c_double
@@ -322,6 +323,7 @@ class Context:
         cfg.staging = 1 if staging else 0
         cfg.device = device
         self.cfg = cfg
+        self.tile_y = 32
         self._h = ctypes.c_void_p()
         _check(lib.lhCreate(ctypes.byref(cfg), ctypes.byref(self._h)), "lhCreate")
         self.lib = lib
@@ -358,6 +360,12 @@ class Context:
         """Backproject only the z-slab [z0, z1) of the volume (one GPU's share
         of a z-sharded volume); multiples of 8."""
         _check(self.lib.lhSetSlab(self._h, z0, z1), "lhSetSlab")
+
+    def set_tile(self, tile_y: int) -> None:
+        """Tile y extent of staged 16-voxel runs: 32 (16 x 32 x 8 voxels, the
+        default), 16 (16 x 16 x 16) or 64 (16 x 64 x 4)."""
+        _check(self.lib.lhSetTile(self._h, tile_y), "lhSetTile")
+        self.tile_y = tile_y
 
     def scale_volume(self, s: float) -> None:
         _check(self.lib.lhScaleVolume(self._h, s), "lhScaleVolume")
@@ -488,10 +496,12 @@ def gather_roof(lds: bool, blocks_per_cu: int = 4, passes: int = 64) -> float:
 
 
 def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel="fused",
-                      views_per_pass=32, run_length=16, volume=None, staging=True, slab=None):
+                      views_per_pass=32, run_length=16, volume=None, staging=True, slab=None,
+                      tile_y=32):
     """Backproject views (N,12)/(N,H,W) on the GPU; returns the (L,L,L) float32
     volume (+= into ``volume`` when given).  ``ranges`` None = unclipped;
-    ``slab`` (z0, z1) restricts the work to that z-slab."""
+    ``slab`` (z0, z1) restricts the work to that z-slab; ``tile_y`` is the
+    staged tile's y extent for 16-voxel runs (16, 32 or 64)."""
     images = np.asarray(images, dtype=np.float32)
     N, H, W = images.shape
     with Context(L, W, H, N, n_slots=N, precision=precision, kernel=kernel,
@@ -499,6 +509,8 @@ def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel=
                  clip=ranges is not None, staging=staging) as ctx:
         if slab is not None:
             ctx.set_slab(*slab)
+        if tile_y != 32:
+            ctx.set_tile(tile_y)
         if ranges is not None:
             ctx.set_ranges(ranges)
         if volume is not None:
