@@ -277,6 +277,103 @@ def cpu_baseline(args, geom, phantom, ranges):
                        "runner's pageable buffers included"}}
 
 
+def _fs_type(path):
+    """File system type of the mount holding path (from /proc/mounts)."""
+    best, kind = "", "unknown"
+    try:
+        with open("/proc/mounts") as f:
+            for line in f:
+                dev, mnt, fs = line.split()[:3]
+                if os.path.abspath(path).startswith(mnt) and len(mnt) > len(best):
+                    best, kind = mnt, fs
+    except OSError:
+        pass
+    return kind
+
+
+def _evict(path):
+    """Drops the file's clean pages from the page cache (a cold read next)."""
+    fd = os.open(path, os.O_RDONLY)
+    try:
+        os.fsync(fd)
+        os.posix_fadvise(fd, 0, 0, os.POSIX_FADV_DONTNEED)
+    finally:
+        os.close(fd)
+
+
+def rct_ingest_baseline(args, geom, phantom):
+    """.rct ingest: the reference runner's input loop (oracle/_ref/
+    rct_read_bench: ctFileReaderReadImage, rabbitRunner.c:266-283, b = 16,
+    into host buffers) against lhLoadRctFile (parallel pread into the pinned
+    upload ring, H2D copies and staging into the context's HBM slots) on the
+    same file of all views, with the page cache warm (just written) and cold
+    (its pages dropped with posix_fadvise before each read)."""
+    from oracle import oracle as orc
+    reader = os.path.join(orc.REF_DIR, "rct_read_bench")
+    if not os.path.exists(reader):
+        log("rct ingest: oracle/_ref/rct_read_bench not built")
+        return None
+    N = geom.n_views
+    imgs = np.empty((N, geom.height, geom.width), np.float32)
+    with lolahip.Context(args.L, geom.width, geom.height, N, n_slots=16) as ctx:
+        for k in range(0, N, 16):
+            generate_into(ctx, geom, phantom, views=np.arange(k, min(k + 16, N)),
+                          want_host=True, out=imgs[k:k + 16], ids=np.arange(k + 1, k + 17))
+    d = tempfile.mkdtemp(prefix="rctio", dir=os.environ.get("RCT_IO_DIR"))
+    path = os.path.join(d, "in.rct")
+    try:
+        hdr = rct.RctHeader(geom.width, geom.height, N, (1000.0, 0.0))
+        rct.write_rct(path, hdr, geom.matrices, imgs)
+        del imgs
+        size = os.path.getsize(path)
+        log(f"rct ingest: {size / 1e9:.2f} GB file on {_fs_type(path)}")
+
+        def ref_read():
+            p = subprocess.run([reader, path, "16"], capture_output=True, text=True,
+                               timeout=600)
+            if p.returncode != 0:
+                raise RuntimeError(p.stdout[-1000:] + p.stderr[-1000:])
+            return json.loads(p.stdout.strip().splitlines()[-1])["seconds"]
+
+        with lolahip.Context(args.L, geom.width, geom.height, N, n_slots=N) as ctx:
+            def lh_read():
+                ctx.synchronize()
+                t = time.perf_counter()
+                ctx.load_rct(path, 0, N)
+                ctx.synchronize()
+                return time.perf_counter() - t
+
+            res = {}
+            for cache in ("warm", "cold"):
+                for name, f in (("reference_fread_loop", ref_read), ("lhLoadRctFile", lh_read)):
+                    best = None
+                    for _ in range(2):
+                        if cache == "cold":
+                            _evict(path)
+                        sec = f()
+                        best = sec if best is None else min(best, sec)
+                    res.setdefault(name, {})[f"{cache}_s"] = round(best, 4)
+                    res[name][f"{cache}_gb_per_s"] = round(size / best / 1e9, 2)
+                    log(f"rct ingest {cache}: {name} {best:.3f} s ({size / best / 1e9:.2f} GB/s)")
+    finally:
+        if os.path.exists(path):
+            os.unlink(path)
+        os.rmdir(d)
+    return {"metric": ".rct ingest, all views of the L=512 workload", "file_bytes": size,
+            "file_system": _fs_type(d), "views": N, "detector": [geom.width, geom.height],
+            "threads": len(os.sched_getaffinity(0)), "cpu_model": cpu_model(),
+            "reference_fread_loop": {**res["reference_fread_loop"],
+                                     "into": "host buffers of 16 projections (runner -b 16)"},
+            "lhLoadRctFile": {**res["lhLoadRctFile"],
+                              "into": "HBM slots (FP32 staging), H2D and staging included"},
+            "cold": "pages dropped with posix_fadvise(DONTNEED) before each read (no effect "
+                    "on tmpfs)",
+            "speedup_warm": round(res["reference_fread_loop"]["warm_s"]
+                                  / res["lhLoadRctFile"]["warm_s"], 2),
+            "speedup_cold": round(res["reference_fread_loop"]["cold_s"]
+                                  / res["lhLoadRctFile"]["cold_s"], 2)}
+
+
 def sparse_cnn(args):
     """Volumes/s of full-view plain vs sparse-view + CNN at L=256."""
     from rabbitct_amd import learned
@@ -453,6 +550,9 @@ def main():
                     help="skip the host-memory pipeline (H2D copies inside the timed step)")
     ap.add_argument("--no-sparse-cnn", action="store_true",
                     help="skip the sparse-view + CNN volumes/s measurement at L=256")
+    ap.add_argument("--rct-io", action="store_true",
+                    help="only time .rct ingest: the reference runner's read loop against "
+                         "lhLoadRctFile on the file of all views; prints that JSON line")
     ap.add_argument("--slab", default="",
                     help="Z0:Z1: backproject only that z-slab (one GPU's share of a volume "
                          "sharded into z-slabs; multiples of 8)")
@@ -468,6 +568,9 @@ def main():
     geom = ScanGeometry(n_views=args.views, seed=SEED).build()
     phantom = EllipsoidPhantom(n_ell=args.ellipsoids, seed=SEED).build()
     log(f"geometry: L={args.L}, {args.views} views, {geom.width}x{geom.height}")
+    if args.rct_io:
+        print(json.dumps(rct_ingest_baseline(args, geom, phantom)), flush=True)
+        return
     ranges = lolahip.compute_line_ranges(args.L, geom.width, geom.height, geom.matrices, 16,
                                          device=True)
     results, vols, lat = {}, {}, {}

@@ -131,7 +131,7 @@ enum {
   LH_E_NODEV = -2,    /* no visible GPU */
   LH_E_NOMEM = -3,    /* device or pinned allocation failed */
   LH_E_HIP = -4,      /* other HIP runtime error */
-  LH_E_IO = -5,       /* clip file read/write/size error */
+  LH_E_IO = -5,       /* file read/write/size error */
   LH_E_STATE = -6     /* call out of order */
 };
 
@@ -438,6 +438,22 @@ int lhHuMetric(const float *dVol, uint64_t n, const uint16_t *dRef, float hus0, 
     double *mse, double *maxAbs, uint32_t *hist);
 /* The same HU conversion into a uint16 volume (device). */
 int lhHuConvert(const float *dVol, uint64_t n, float hus0, float hus1, uint16_t *dOut);
+
+/* The runner's -p slice image (rabbitRunner.c:292-315): a 16-bit binary PGM,
+ * header "P5\n<L> <L>\n4095\n", then the HU values (lhHuConvert) of slice
+ * z = L/2 - 1 of the L^3 device volume as big-endian uint16. */
+int lhWritePgmSlice(const char *path, uint32_t L, const float *dVol, float hus0, float hus1);
+
+/* The runner's -c result file (rabbitRunner.c:318-407, result.rctd): int32
+ * 591984, int32 L, uint32 numberOfImages, numberOfImages x uint64
+ * meanTimeUsec (the runner stores its mean time per image for every image),
+ * the HU slice z = L/2 - 1 as L x L floats, double mse, double max |error|,
+ * uint32 hist[11] (lhHuMetric of the L^3 device volume against the device
+ * uint16 reference), int32 489195; little-endian, as the runner writes on
+ * x86.  mse / maxAbs / hist (optional) receive the metric. */
+int lhWriteResultFile(const char *path, uint32_t L, uint32_t numberOfImages,
+    uint64_t meanTimeUsec, const float *dVol, const uint16_t *dRef, float hus0, float hus1,
+    double *mse, double *maxAbs, uint32_t *hist);
 
 /* Last error message of the calling thread. */
 const char *lhLastError(void);

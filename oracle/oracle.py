@@ -149,6 +149,30 @@ def hu_convert(vol, h0, h1):
     return out
 
 
+def pgm_slice_bytes(vol, h0, h1):
+    """The runner's -p file for an (L,L,L) volume (rabbitRunner.c:292-315):
+    "P5\n<L> <L>\n4095\n", then the HU of slice z = L/2-1 as big-endian
+    uint16."""
+    L = vol.shape[0]
+    hu = hu_convert(vol[L // 2 - 1], h0, h1)
+    return f"P5\n{L} {L}\n4095\n".encode() + hu.astype(">u2").tobytes()
+
+
+def result_rctd_bytes(vol, ref_u16, h0, h1, n_images, mean_time_us):
+    """The runner's result.rctd for an (L,L,L) volume against a uint16
+    reference (rabbitRunner.c:318-407): magic 591984, L, the image count,
+    the mean time per image once per image (uint64), the HU slice
+    z = L/2-1 as floats, mse, max |error| (doubles), the 11-bin error
+    histogram (uint32), magic 489195; x86 byte order."""
+    L = vol.shape[0]
+    m = hu_metric(vol, ref_u16, h0, h1)
+    hu = hu_convert(vol[L // 2 - 1], h0, h1).astype(np.float32)
+    return b"".join([np.array([591984, L], "<i4").tobytes(), np.array([n_images], "<u4").tobytes(),
+                     np.full(n_images, mean_time_us, "<u8").tobytes(), hu.astype("<f4").tobytes(),
+                     np.array([m["mse"], m["max"]], "<f8").tobytes(),
+                     m["hist"].astype("<u4").tobytes(), np.array([489195], "<i4").tobytes()])
+
+
 def line_integrals(src, minv, ellipsoids, W, H):
     """FP64 analytic line integrals (N,H,W) of an ellipsoid phantom."""
     src = _c(src, np.float64)

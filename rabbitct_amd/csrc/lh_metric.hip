@@ -7,6 +7,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <vector>
 
 #include "lolahip.h"
 #include "lh_internal.h"
@@ -65,6 +67,32 @@ __global__ __launch_bounds__(256) void hu_metric_kernel(const float *__restrict_
   }
 }
 
+// HU slice z = L/2 - 1 of a device volume (the slice the runner's PGM and
+// result.rctd hold, rabbitRunner.c:298,400) on the host.
+int hu_slice(const float *dVol, uint32_t L, float h0, float h1, std::vector<uint16_t> &out)
+{
+  const uint64_t ss = (uint64_t)L * L;
+  out.resize(ss);
+  uint16_t *d   = nullptr;
+  hipError_t e = hipMalloc(&d, ss * sizeof(uint16_t));
+  if (e != hipSuccess)
+    return lh_fail(LH_E_HIP, "HU slice: %s", hipGetErrorString(e));
+  int rc = lhHuConvert(dVol + (uint64_t)(L / 2 - 1) * ss, ss, h0, h1, d);
+  if (rc == LH_OK) {
+    e = hipMemcpy(out.data(), d, ss * sizeof(uint16_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+      rc = lh_fail(LH_E_HIP, "HU slice: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+// fwrite of n items or a failure status
+bool put(FILE *f, const void *p, size_t size, size_t n)
+{
+  return fwrite(p, size, n, f) == n;
+}
+
 } // namespace
 
 extern "C" {
@@ -109,6 +137,64 @@ int lhHuMetric(const float *dVol, uint64_t n, const uint16_t *dRef, float hus0, 
   if (hist)
     for (int b = 0; b < 11; ++b)
       hist[b] = (uint32_t)h[2 + b];
+  return LH_OK;
+}
+
+int lhWritePgmSlice(const char *path, uint32_t L, const float *dVol, float hus0, float hus1)
+{
+  if (!path || !dVol || L < 2)
+    return lh_fail(LH_E_ARG, "lhWritePgmSlice: bad arguments");
+  std::vector<uint16_t> hu;
+  const int rc = hu_slice(dVol, L, hus0, hus1, hu);
+  if (rc != LH_OK)
+    return rc;
+  // 16-bit binary PGM: samples big-endian
+  for (uint16_t &v : hu)
+    v = (uint16_t)((v >> 8) | (v << 8));
+  FILE *f = fopen(path, "wb");
+  if (!f)
+    return lh_fail(LH_E_IO, "lhWritePgmSlice: cannot create %s", path);
+  bool ok = fprintf(f, "P5\n%u %u\n4095\n", L, L) > 0 && put(f, hu.data(), 2, hu.size());
+  ok = fclose(f) == 0 && ok;
+  return ok ? LH_OK : lh_fail(LH_E_IO, "lhWritePgmSlice: write to %s failed", path);
+}
+
+int lhWriteResultFile(const char *path, uint32_t L, uint32_t numberOfImages,
+    uint64_t meanTimeUsec, const float *dVol, const uint16_t *dRef, float hus0, float hus1,
+    double *mse, double *maxAbs, uint32_t *hist)
+{
+  if (!path || !dVol || !dRef || L < 2)
+    return lh_fail(LH_E_ARG, "lhWriteResultFile: bad arguments");
+  const uint64_t n = (uint64_t)L * L * L;
+  double err = 0.0, mx = 0.0;
+  uint32_t h[11];
+  int rc = lhHuMetric(dVol, n, dRef, hus0, hus1, &err, &mx, h);
+  std::vector<uint16_t> hu;
+  if (rc == LH_OK)
+    rc = hu_slice(dVol, L, hus0, hus1, hu);
+  if (rc != LH_OK)
+    return rc;
+  // the slice as the runner holds it after its -c loop: HU values as floats
+  std::vector<float> slice(hu.begin(), hu.end());
+  const std::vector<uint64_t> times(numberOfImages, meanTimeUsec);
+  const int32_t magic = 591984, magicEnd = 489195, size = (int32_t)L;
+  FILE *f = fopen(path, "wb");
+  if (!f)
+    return lh_fail(LH_E_IO, "lhWriteResultFile: cannot create %s", path);
+  bool ok = put(f, &magic, 4, 1) && put(f, &size, 4, 1) && put(f, &numberOfImages, 4, 1) &&
+            put(f, times.data(), 8, times.size()) && put(f, slice.data(), 4, slice.size()) &&
+            put(f, &err, 8, 1) && put(f, &mx, 8, 1) && put(f, h, 4, 11) &&
+            put(f, &magicEnd, 4, 1);
+  ok = fclose(f) == 0 && ok;
+  if (!ok)
+    return lh_fail(LH_E_IO, "lhWriteResultFile: write to %s failed", path);
+  if (mse)
+    *mse = err;
+  if (maxAbs)
+    *maxAbs = mx;
+  if (hist)
+    for (int b = 0; b < 11; ++b)
+      hist[b] = h[b];
   return LH_OK;
 }
 

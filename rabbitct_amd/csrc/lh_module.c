@@ -368,9 +368,11 @@ int lolaHipFinish(RabbitCtGlobalData *rcgd)
  * runner reads each record into its own buffers, ctFileReader.c:47-61): views
  * [firstView, firstView+count) are read with pread by all host threads into
  * the context's pinned upload ring and committed to slots firstSlot.., so the
- * file reads overlap the H2D copies and staging of earlier views.  The
- * record layout is ctFileReader.c:51-58.  matrices (optional, count x 12)
- * receives the views' matrices.
+ * file reads overlap the H2D copies and staging of earlier views; the kernel
+ * is asked to read the next kReadAhead records ahead (POSIX_FADV_WILLNEED),
+ * so a file not in the page cache streams in while earlier views are copied.
+ * The record layout is ctFileReader.c:51-58.  matrices (optional,
+ * count x 12) receives the views' matrices.
  * ------------------------------------------------------------------------- */
 static int pread_all(int fd, char *dst, size_t len, off_t off)
 {
@@ -387,6 +389,11 @@ static int pread_all(int fd, char *dst, size_t len, off_t off)
   }
   return 0;
 }
+
+#ifndef LH_READ_AHEAD
+#define LH_READ_AHEAD 32
+#endif
+enum { kReadAhead = LH_READ_AHEAD };
 
 int lhLoadRctFile(LhContext *ctx, const char *path, uint32_t firstView, uint32_t count,
     uint32_t firstSlot, double *matrices)
@@ -410,8 +417,16 @@ int lhLoadRctFile(LhContext *ctx, const char *path, uint32_t firstView, uint32_t
   }
   const size_t imgBytes = (size_t)cfg.W * cfg.H * sizeof(float);
   const size_t record   = 12 * sizeof(double) + imgBytes;
+  uint32_t advised      = 0; /* records [0, advised) asked for */
   for (uint32_t v = 0; rc == LH_OK && v < count; v++) {
     const off_t off = (off_t)20 + (off_t)(firstView + v) * (off_t)record;
+    if (advised < count && advised < v + kReadAhead / 2) {
+      /* refilled to kReadAhead records ahead each time fewer than half remain */
+      const uint32_t end = v + kReadAhead < count ? v + kReadAhead : count;
+      (void)posix_fadvise(fd, (off_t)20 + (off_t)(firstView + advised) * (off_t)record,
+          (off_t)(end - advised) * (off_t)record, POSIX_FADV_WILLNEED);
+      advised = end;
+    }
     double mat[12];
     float *buf = NULL;
     if (pread_all(fd, (char *)mat, sizeof mat, off) != 0) {

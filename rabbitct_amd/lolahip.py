@@ -83,7 +83,7 @@ EXPORTS = (
     "lhConv3dSplitTest",
     "lhDeviceAlloc", "lhDeviceFree", "lhCopyToDevice", "lhCopyToHost",
     # before and after the path
-    "lhFdkFilter", "lhHuMetric", "lhHuConvert", "lhGetConfig", "lhLoadRctFile",
+    "lhFdkFilter", "lhHuMetric", "lhHuConvert", "lhWritePgmSlice", "lhWriteResultFile", "lhGetConfig", "lhLoadRctFile",
     # host-memory reconstruction and the module's timing contract
     "lhQueryIdle", "lhHostAlloc", "lhHostFree", "lhReconstructHost", "lhTimedReconstructHost",
     "lolaHipQueryIdle", "lhDeviceInfo", "lhTimedUploadHost",
@@ -174,6 +174,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhFdkFilter.argtypes = [i32, i32, u32, P, P, P, f64, f64, i32, P, P]
     lib.lhHuMetric.argtypes = [P, u64, P, f32, f32, ctypes.POINTER(f64), ctypes.POINTER(f64), P]
     lib.lhHuConvert.argtypes = [P, u64, f32, f32, P]
+    lib.lhWritePgmSlice.argtypes = [ctypes.c_char_p, u32, P, f32, f32]
+    lib.lhWriteResultFile.argtypes = [ctypes.c_char_p, u32, u32, u64, P, P, f32, f32,
+                                      ctypes.POINTER(f64), ctypes.POINTER(f64), P]
     lib.lhGetConfig.argtypes = [P, ctypes.POINTER(LhConfig)]
     lib.lhLoadRctFile.argtypes = [P, ctypes.c_char_p, u32, u32, u32, P]
     lib.lhQueryIdle.argtypes = [P]

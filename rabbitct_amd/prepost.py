@@ -1,10 +1,13 @@
 """The steps either side of the backprojection, on the GPU: FDK
 pre-processing of measured line integrals (the RabbitCT input carries its
-result) and the runner's -c verification metric (rabbitRunner.c:346-384)."""
+result), the runner's -c verification metric (rabbitRunner.c:346-384) and
+its output files (the -p PGM slice, rabbitRunner.c:292-315, and
+result.rctd, rabbitRunner.c:396-407)."""
 from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -45,3 +48,24 @@ def hu_metric(d_vol, d_ref, n: int, hus0: float, hus1: float) -> dict:
 def hu_convert(d_vol, n: int, hus0: float, hus1: float, d_out) -> None:
     lib = lolahip.load_library()
     _check(lib.lhHuConvert(unet._addr(d_vol), n, hus0, hus1, unet._addr(d_out)), "lhHuConvert")
+
+
+def write_pgm_slice(path, L: int, d_vol, hus0: float, hus1: float) -> None:
+    """The runner's -p image of slice z = L/2-1 of the device volume."""
+    lib = lolahip.load_library()
+    _check(lib.lhWritePgmSlice(os.fsencode(str(path)), L, unet._addr(d_vol), hus0, hus1),
+           "lhWritePgmSlice")
+
+
+def write_result_file(path, L: int, n_images: int, mean_time_us: int, d_vol, d_ref,
+                      hus0: float, hus1: float) -> dict:
+    """The runner's result.rctd for the device volume against the device
+    uint16 reference; returns the -c metric it holds."""
+    lib = lolahip.load_library()
+    mse, mx = ctypes.c_double(), ctypes.c_double()
+    hist = np.zeros(11, np.uint32)
+    _check(lib.lhWriteResultFile(os.fsencode(str(path)), L, n_images, mean_time_us,
+                                 unet._addr(d_vol), unet._addr(d_ref), hus0, hus1,
+                                 ctypes.byref(mse), ctypes.byref(mx), _p(hist)),
+           "lhWriteResultFile")
+    return {"mse": mse.value, "max": mx.value, "hist": hist}

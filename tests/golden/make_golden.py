@@ -19,10 +19,13 @@ covers the same field as 1248 x 960 x 0.3 mm; 12 views on the 200-degree arc):
   shadow           the padding LolaASM prints (bottom top left right, VS 16)
   pgm_slice        the HU slice the runner writes with -p for opt16
                    (rabbitRunner.c:292-315): pins the HU conversion
+  pgm_file         that PGM file's bytes (header and big-endian samples)
   check_*          what the runner prints for -c (mse, max, psnr) and writes to
                    result.rctd (hist) when Reference128.vol is the runner's own
                    HU volume plus pert_val at pert_idx: the HU error is then
                    exactly -pert_val there and 0 elsewhere
+  check_rctd       the bytes of that run's result.rctd (rabbitRunner.c:396-407;
+                   its per-image times are the run's own measurement)
 """
 from __future__ import annotations
 
@@ -99,6 +102,7 @@ def make(border: bool) -> dict:
                 opt16 = vol
                 raw = open(os.path.join(d, "slice.pgm"), "rb").read()
                 out["pgm_slice"] = np.frombuffer(raw[-2 * L * L:], dtype=">u2").astype(np.uint16).reshape(L, L)
+                out["pgm_file"] = np.frombuffer(raw, dtype=np.uint8).copy()
         out["clip16"] = rct.read_clip_file(os.path.join(d, "clip16.rct"), N, L)
         out["clip8"] = rct.read_clip_file(os.path.join(d, "clip8.rct"), N, L)
         # -c: the reference volume is the runner's own HU volume, perturbed
@@ -120,6 +124,7 @@ def make(border: bool) -> dict:
         raw = open(os.path.join(d, "result.rctd"), "rb").read()
         hist_off = 4 + 4 + 4 + 8 * N + 4 * L * L + 8 + 8
         out["check_hist"] = np.frombuffer(raw[hist_off:hist_off + 44], dtype=np.uint32).copy()
+        out["check_rctd"] = np.frombuffer(raw, dtype=np.uint8).copy()
     return out
 
 

@@ -223,3 +223,12 @@ def test_host_batch_plan_covers_the_views_with_short_ends():
         if count > 2 * batch and batch >= 4:
             assert plan[0] == batch // 4 and plan[-1] == batch // 4
     assert lolahip.host_batch_plan(496, 64) == [16] + [64] * 7 + [16, 16]
+
+
+def test_writers_reject_bad_arguments_before_any_gpu_call():
+    lib = lolahip.load_library()
+    LH_E_ARG = -1
+    assert lib.lhWritePgmSlice(b"x.pgm", 1, ctypes.c_void_p(8), 1.0, 0.0) == LH_E_ARG
+    assert lib.lhWriteResultFile(None, 128, 1, 0, None, None, 1.0, 0.0, None, None, None) \
+        == LH_E_ARG
+    assert not os.path.exists("x.pgm")

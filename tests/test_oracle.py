@@ -269,3 +269,35 @@ def test_backproject_lines_equals_clipped_volume(golden_border):
     lines = orc.backproject_lines(L, voxel_size(L), volume_origin(L), g["matrices"],
                                   g["images"], g["clip16"], zs, ys)
     np.testing.assert_array_equal(lines, full[zs, ys, :])
+
+
+def _runner_check_case(golden):
+    """A volume whose slice z = L/2-1 is the runner's LolaOPT slice, and the
+    reference HU(volume) + pert: the runner's -c outputs for its own volume
+    against HU(own) + pert depend only on that slice and on pert."""
+    rng = np.random.default_rng(3)
+    vol = rng.uniform(-0.5, 3.0, (L, L, L)).astype(np.float32)
+    vol[L // 2 - 1] = golden["opt16_slice"]
+    h0, h1 = golden["hu"]
+    ref = orc.hu_convert(vol, h0, h1).reshape(-1)
+    ref[golden["pert_idx"]] += golden["pert_val"]
+    raw = golden["check_rctd"].tobytes()
+    n_images = int(np.frombuffer(raw[8:12], "<u4")[0])
+    mean_time = int(np.frombuffer(raw[12:20], "<u8")[0])
+    return vol, ref, h0, h1, n_images, mean_time
+
+
+def test_result_rctd_restatement_byte_equal_runner(golden):
+    vol, ref, h0, h1, n_images, mean_time = _runner_check_case(golden)
+    raw = golden["check_rctd"].tobytes()
+    assert n_images == len(golden["matrices"])
+    got = orc.result_rctd_bytes(vol, ref, h0, h1, n_images, mean_time)
+    assert len(got) == len(raw)
+    assert got == raw
+
+
+def test_pgm_restatement_byte_equal_runner(golden):
+    h0, h1 = golden["hu"]
+    vol = np.zeros((L, L, L), np.float32)
+    vol[L // 2 - 1] = golden["opt16_slice"]
+    assert orc.pgm_slice_bytes(vol, h0, h1) == golden["pgm_file"].tobytes()

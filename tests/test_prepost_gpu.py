@@ -106,3 +106,37 @@ def test_fdk_cosine_weight_and_ramp_taps_closed_form():
                 assert np.all(out[:, j, i + k] == 0.0)
     # the principal ray: weight 1
     np.testing.assert_allclose(out[:, 40, 52] * 4 * tau / dbeta, 1.0, rtol=1e-6)
+
+
+def test_result_file_byte_equal_runner(golden, tmp_path):
+    """lhWriteResultFile writes the runner's result.rctd byte for byte: the
+    volume's slice z = L/2-1 is the runner's LolaOPT slice and the reference
+    is HU(volume) + pert, so the runner's own run (its volume against HU(own)
+    + pert) wrote the same bytes; its measured mean time per image is passed
+    through."""
+    rng = np.random.default_rng(3)
+    vol = rng.uniform(-0.5, 3.0, (L, L, L)).astype(np.float32)
+    vol[L // 2 - 1] = golden["opt16_slice"]
+    h0, h1 = golden["hu"]
+    ref = orc.hu_convert(vol, h0, h1).reshape(-1)
+    ref[golden["pert_idx"]] += golden["pert_val"]
+    raw = golden["check_rctd"].tobytes()
+    n_images = int(np.frombuffer(raw[8:12], "<u4")[0])
+    mean_time = int(np.frombuffer(raw[12:20], "<u8")[0])
+    path = tmp_path / "result.rctd"
+    with unet.DeviceBuffer.from_array(vol) as dv, unet.DeviceBuffer.from_array(ref) as dr:
+        m = prepost.write_result_file(path, L, n_images, mean_time, dv, dr, h0, h1)
+    assert path.read_bytes() == raw
+    assert m["max"] == float(golden["check_max"])
+    assert np.array_equal(m["hist"], golden["check_hist"])
+
+
+def test_pgm_file_byte_equal_runner(golden, tmp_path):
+    h0, h1 = golden["hu"]
+    vol = np.zeros((L, L, L), np.float32)
+    vol[L // 2 - 1] = golden["opt16_slice"]
+    path = tmp_path / "slice.pgm"
+    with unet.DeviceBuffer.from_array(vol) as dv:
+        prepost.write_pgm_slice(path, L, dv, h0, h1)
+    assert path.read_bytes() == golden["pgm_file"].tobytes()
+
