@@ -44,13 +44,11 @@ constexpr int kRsrcWord3 = 0x00020000;
 constexpr uint32_t kMissOffset = 0x80000000u;
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// FP32 -> bfloat16, round to nearest even (one v_cvt_pk_bf16_f32; two
+// values share one when the compiler pairs them)
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
-  uint32_t b = __float_as_uint(f);
-  if ((b & 0x7fffffffu) > 0x7f800000u)
-    return (uint16_t)((b >> 16) | 0x40u);
-  b += 0x7fffu + ((b >> 16) & 1u);
-  return (uint16_t)(b >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 // Number of 32-wide K chunks and the chunk/entry of (tap, ci) for C input
@@ -123,6 +121,49 @@ __device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, 
   constexpr int TOTAL = HV * Q;
   constexpr int ITER  = (TOTAL + 255) / 256;
   const int D = a.D;
+  // A pass whose channels all come from one tensor (the usual case: the
+  // decoder's splits fall on pass boundaries) selects that tensor once, so
+  // the buffer descriptor is wave-uniform; a pass that straddles the split
+  // selects per 8-channel unit.
+  const bool straddle = cbase < a.c0 && cbase + CC > a.c0;
+  if (!straddle) {
+    const bool first  = cbase < a.c0;
+    const ConvSrc &sr = first ? a.s0 : a.s1;
+    const int half    = sr.half;
+    const int Ds      = half ? (D >> 1) : D;
+    const uint32_t ld = (uint32_t)sr.ld;
+    const uint32_t c0 = (uint32_t)(sr.ofs + (first ? cbase : cbase - a.c0));
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+#pragma unroll 1
+    for (int b0 = 0; b0 < ITER; b0 += kStageBatch) {
+      u32x4 val[kStageBatch];
+#pragma unroll
+      for (int k = 0; k < kStageBatch; ++k) {
+        const int i = threadIdx.x + (b0 + k) * 256;
+        if (b0 + k < ITER && i < TOTAL) {
+          const int q = i % Q, v = i / Q;
+          const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+          const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+          const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+                          (unsigned)gz < (unsigned)D;
+          const int sx = half ? (gx >> 1) : gx, sy = half ? (gy >> 1) : gy,
+                    sz = half ? (gz >> 1) : gz;
+          const uint32_t off =
+              in ? ((((uint32_t)n * Ds + sz) * Ds + sy) * Ds + sx) * ld * 2u + (c0 + 8 * q) * 2u
+                 : kMissOffset;
+          val[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kStageBatch; ++k) {
+        const int i = threadIdx.x + (b0 + k) * 256;
+        if (b0 + k < ITER && i < TOTAL)
+          *reinterpret_cast<u32x4 *>(lds + (i / Q) * CC + 8 * (i % Q)) = val[k];
+      }
+    }
+    return;
+  }
 #pragma unroll 1
   for (int b0 = 0; b0 < ITER; b0 += kStageBatch) {
     u32x4 val[kStageBatch];
@@ -220,6 +261,166 @@ template <int NT, int MT> constexpr size_t conv_lds_bytes()
   const size_t halo = (size_t)(MT + 2) * kHY * kHX * 32 * 2;
   const size_t outb = (size_t)4 * MT * 16 * NT * 16 * 2;
   return halo > outb ? halo : outb;
+}
+
+// The 16-channel pass on a staged halo [MT+2][kHY][kHX][16] (K chunks
+// k0 + 5 dz + j: lanes kg < 2 hold tap (dz, 2j), kg >= 2 tap (dz, 2j+1), each
+// lane 8 of the 16 channels), input planes outer: the A operand of halo plane
+// p at (dy, dx) serves output rows p - dz for dz = 0, 1, 2.
+template <int NT, int MT>
+__device__ __forceinline__ void tail16_mfma(f32x4 (&acc)[MT][NT], const uint16_t *lds,
+    const uint16_t *wrow, size_t wtile, int k0, int w, int m, int kg)
+{
+  constexpr int RS16 = kHX * kHY * 16;
+  const int half = kg >> 1, cofs = 8 * (kg & 1);
+  bf16x8 Bn[3][NT];
+#pragma unroll
+  for (int dz = 0; dz < 3; ++dz)
+    load_b(Bn[dz], wrow, wtile, k0 + 5 * dz);
+  for (int j = 0; j < 5; ++j) {
+    bf16x8 B[3][NT];
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        B[dz][t] = Bn[dz][t];
+    if (j + 1 < 5)
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz)
+        load_b(Bn[dz], wrow, wtile, k0 + 5 * dz + j + 1);
+    // e = 9 is the pad: it reads tap 8's operand, whose products meet the
+    // pad's weights, zero in the packing (finite activations)
+    const int e  = 2 * j + half;
+    const int ee = e >= 9 ? 8 : e;
+    const uint16_t *base = lds + ((w + ee / 3) * kHX + m + ee % 3) * 16 + cofs;
+#pragma unroll
+    for (int p = 0; p < MT + 2; ++p) {
+      const bf16x8 A =
+          __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + p * RS16));
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const int r = p - dz;
+        if (r >= 0 && r < MT)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[dz][t], acc[r][t], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Epilogue of a convolution tile: bias, ReLU, bfloat16 into an LDS transpose
+// buffer (this wave's rows: [MT][16 voxels][COUT]), then 16-byte stores of 8
+// channels; the fused pooling and 1x1x1 head read the same buffer.  Begins
+// with a barrier: every wave's reads of the tile's input from lds are done.
+template <int NT, int MT>
+__device__ __forceinline__ void conv_epilogue(const ConvLdsArgs &a, f32x4 (&acc)[MT][NT],
+    uint16_t *lds, int n, int x0, int y0, int z0, int w, int lane, int m, int kg)
+{
+  const int D = a.D;
+  constexpr int COUT = NT * 16;
+  __syncthreads();
+  uint16_t *ob = lds + (size_t)w * MT * 16 * COUT;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = 16 * t + m;
+    const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < MT; ++r)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float val = acc[r][t][i] + bias;
+        if (a.relu)
+          val = fmaxf(val, 0.0f);
+        ob[(r * 16 + 4 * kg + i) * COUT + co] = f2bf(val);
+      }
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): this wave's LDS writes are done
+  __builtin_amdgcn_wave_barrier();
+  constexpr int UNITS = MT * 16 * COUT / 8;  // 16-byte units of this wave
+  if (a.y) {
+#pragma unroll
+    for (int u0 = 0; u0 < UNITS; u0 += 64) {
+      const int u = u0 + lane;
+      if (UNITS % 64 == 0 || u < UNITS) {
+        const int c8 = u % (COUT / 8), vx = (u / (COUT / 8)) % 16, r = u / (COUT / 8 * 16);
+        uint4 val = *reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * COUT + 8 * c8);
+        const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
+        uint4 *dst = reinterpret_cast<uint4 *>(a.y + vo * a.ldy + a.yofs + 8 * c8);
+        if (a.accumulate) {
+          const uint4 old = *dst;
+          const uint32_t nv[4] = { val.x, val.y, val.z, val.w };
+          const uint32_t ov[4] = { old.x, old.y, old.z, old.w };
+          uint32_t rv[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            rv[k] = (uint32_t)f2bf(__uint_as_float(nv[k] << 16) + __uint_as_float(ov[k] << 16)) |
+                    ((uint32_t)f2bf(__uint_as_float(nv[k] & 0xffff0000u) +
+                                    __uint_as_float(ov[k] & 0xffff0000u)) << 16);
+          val = make_uint4(rv[0], rv[1], rv[2], rv[3]);
+        }
+        *dst = val;
+      }
+    }
+  }
+  if (a.pool) {
+    // the tile (16 x 4 x MT, all even) holds whole 2x2x2 cells; wave w's rows
+    // are at y0 + w, so a cell reads the buffers of waves 2Y and 2Y + 1
+    __syncthreads();
+    constexpr int C8 = COUT / 8;
+    constexpr int CELLS = 8 * 2 * (MT / 2);     // x cells, y cells, z cells
+    const int Dh = D >> 1;
+    for (int u = threadIdx.x; u < CELLS * C8; u += 256) {
+      const int c8 = u % C8, cell = u / C8;
+      const int cx = cell % 8, cy = (cell / 8) % 2, cz = cell / 16;
+      float sum[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int zz = 2 * cz + (q >> 2), yy = 2 * cy + ((q >> 1) & 1), xx = 2 * cx + (q & 1);
+        const uint4 v = *reinterpret_cast<const uint4 *>(lds + (size_t)yy * MT * 16 * COUT +
+                                                        (zz * 16 + xx) * COUT + 8 * c8);
+        const uint32_t w4[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          sum[2 * k] += __uint_as_float(w4[k] << 16);
+          sum[2 * k + 1] += __uint_as_float(w4[k] & 0xffff0000u);
+        }
+      }
+      uint32_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        o[k] = (uint32_t)f2bf(0.125f * sum[2 * k]) | ((uint32_t)f2bf(0.125f * sum[2 * k + 1]) << 16);
+      const size_t vo = ((((size_t)n * Dh + (z0 >> 1) + cz) * Dh + (y0 >> 1) + cy) * Dh) +
+                        (x0 >> 1) + cx;
+      *reinterpret_cast<uint4 *>(a.pool + vo * a.ldp + a.pofs + 8 * c8) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
+  if constexpr (COUT == 16) {
+    if (a.hout) {
+      // one voxel per lane: its 16 channels (bfloat16) dotted with the head
+#pragma unroll
+      for (int v0 = 0; v0 < MT * 16; v0 += 64) {
+        const int vv = v0 + lane;
+        if ((MT * 16) % 64 == 0 || vv < MT * 16) {
+          const int vx = vv % 16, r = vv / 16;
+          const uint4 *hv = reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * 16);
+          const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
+          float sum = a.hin[vo] + a.hb[0];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const uint4 hq = hv[q];
+            const uint32_t hw4[4] = { hq.x, hq.y, hq.z, hq.w };
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              sum = fmaf(a.hw[8 * q + 2 * k], __uint_as_float(hw4[k] << 16), sum);
+              sum = fmaf(a.hw[8 * q + 2 * k + 1], __uint_as_float(hw4[k] & 0xffff0000u), sum);
+            }
+          }
+          a.hout[vo] = sum;
+        }
+      }
+    }
+  }
 }
 
 // Input channels are staged in passes: F = cin/32 passes of 32 channels
@@ -365,42 +566,7 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
     // (dz, 2j+1) (e = 9 is the zero pad); each lane reads 8 of the 16 channels
     const int half = kg >> 1, cofs = 8 * (kg & 1);
     if constexpr (NT <= 2) {
-      // input planes outer, as in the 32-channel passes
-      bf16x8 Bn[3][NT];
-#pragma unroll
-      for (int dz = 0; dz < 3; ++dz)
-        load_b(Bn[dz], wrow, wtile, 27 * F + 5 * dz);
-      for (int j = 0; j < 5; ++j) {
-        bf16x8 B[3][NT];
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-            B[dz][t] = Bn[dz][t];
-        if (j + 1 < 5)
-#pragma unroll
-          for (int dz = 0; dz < 3; ++dz)
-            load_b(Bn[dz], wrow, wtile, 27 * F + 5 * dz + j + 1);
-        const int e = 2 * j + half;
-        const bool pad = e >= 9;
-        const int ee = pad ? 8 : e;        // the pad reads a valid address, zeroed below
-        const uint16_t *base = lds + ((w + ee / 3) * kHX + m + ee % 3) * 16 + cofs;
-#pragma unroll
-        for (int p = 0; p < MT + 2; ++p) {
-          uint4 av = *reinterpret_cast<const uint4 *>(base + p * RS16);
-          if (pad)
-            av = make_uint4(0, 0, 0, 0);
-          const bf16x8 A = __builtin_bit_cast(bf16x8, av);
-#pragma unroll
-          for (int dz = 0; dz < 3; ++dz) {
-            const int r = p - dz;
-            if (r >= 0 && r < MT)
-#pragma unroll
-              for (int t = 0; t < NT; ++t)
-                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[dz][t], acc[r][t], 0, 0, 0);
-          }
-        }
-      }
+      tail16_mfma<NT, MT>(acc, lds, wrow, wtile, 27 * F, w, m, kg);
     } else {
       bf16x8 Bn[NT];
       load_b(Bn, wrow, wtile, 27 * F);
@@ -412,16 +578,13 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
             B[t] = Bn[t];
           if (5 * dz + j + 1 < 15)
             load_b(Bn, wrow, wtile, 27 * F + 5 * dz + j + 1);
-          const int e = 2 * j + half;
-          const bool pad = e >= 9;
-          const int ee = pad ? 8 : e;
+          const int e  = 2 * j + half;
+          const int ee = e >= 9 ? 8 : e;     // the pad: zero weights, as above
           const uint16_t *base = lds + ((dz * kHY + w + ee / 3) * kHX + m + ee % 3) * 16 + cofs;
 #pragma unroll
           for (int r = 0; r < MT; ++r) {
-            uint4 av = *reinterpret_cast<const uint4 *>(base + r * RS16);
-            if (pad)
-              av = make_uint4(0, 0, 0, 0);
-            const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+            const bf16x8 A =
+                __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + r * RS16));
 #pragma unroll
             for (int t = 0; t < NT; ++t)
               acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
@@ -430,110 +593,111 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       }
     }
   }
-  // epilogue: bias, ReLU, bfloat16 into an LDS transpose buffer (this wave's
-  // rows: [MT][16 voxels][COUT]), then 16-byte stores of 8 channels
-  constexpr int COUT = NT * 16;
-  __syncthreads();
-  uint16_t *ob = lds + (size_t)w * MT * 16 * COUT;
+  conv_epilogue<NT, MT>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg);
+}
+
+// Persistent form of the 16-input-channel convolution (one 16-channel pass;
+// a single full-resolution source, a.s0).  A grid sized to the chip walks the tiles
+// (8 contiguous ranges, one per XCD); each tile's halo is staged by
+// direct-to-LDS buffer loads (unit i of [MT+2][kHY][kHX][16] lands 16 bytes
+// at lds + 8 i, 1 KB per wave instruction; units outside the volume read past
+// the buffer range and land zeros) into one of two LDS buffers while the
+// previous tile computes and runs its epilogue in the other.
+template <int MT>
+__device__ __forceinline__ void stage16_dma(const ConvLdsArgs &a, uint16_t *buf, int n, int x0,
+    int y0, int z0)
+{
+  constexpr int TOTAL = kHX * kHY * (MT + 2) * 2;  // 16-byte units
+  constexpr int ITER  = (TOTAL + 255) / 256;
+  const int D         = a.D;
+  const ConvSrc &sr   = a.s0;                      // channels [0, c0) = all 16
+  const uint32_t ld   = (uint32_t)sr.ld;
+  const uint32_t c0   = (uint32_t)sr.ofs;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+  const int wbase = threadIdx.x & ~63;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = 16 * t + m;
-    const float bias = a.bias ? a.bias[co] : 0.0f;
+  for (int k = 0; k < ITER; ++k) {
+    const int i = threadIdx.x + k * 256;
+    if (i < TOTAL) {
+      const int q = i & 1, v = i >> 1;
+      const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+      const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+      const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+                      (unsigned)gz < (unsigned)D;
+      const uint32_t off =
+          in ? ((((uint32_t)n * D + gz) * D + gy) * D + gx) * ld * 2u + (c0 + 8 * q) * 2u
+             : kMissOffset;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + 8 * (wbase + k * 256)), 16, (int)off,
+          0, 0, 0);
+    }
+  }
+}
+
+template <int NT, int MT>
+constexpr int conv16_buf_elems()
+{
+  const int halo = (MT + 2) * kHY * kHX * 16;
+  const int outb = 4 * MT * 16 * NT * 16;
+  return halo > outb ? halo : outb;
+}
+
+template <int NT, int MT>
+__global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
+{
+  extern __shared__ __align__(16) uint16_t lds[];
+  constexpr int BUF = conv16_buf_elems<NT, MT>();
+  const int D = a.D;
+  const int txN = D / 16, tyN = D / 4, tzN = D / MT;
+  const int nTiles = a.n * txN * tyN * tzN;
+  const int perXcd = (nTiles + 7) / 8;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int stride = (gridDim.x + 7 - xcd) / 8;   // workgroups of this XCD
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
+  const size_t wtile = (size_t)16 * a.kp;
+  auto coords = [&](int t, int &n, int &x0, int &y0, int &z0) {
+    x0 = (t % txN) * 16;
+    int rest = t / txN;
+    y0 = (rest % tyN) * 4;
+    rest /= tyN;
+    z0 = (rest % tzN) * MT;
+    n  = rest / tzN;
+  };
+  int k = local, t = xcd * perXcd + k;
+  if (k >= perXcd || t >= nTiles)
+    return;
+  int n, x0, y0, z0;
+  coords(t, n, x0, y0, z0);
+  stage16_dma<MT>(a, lds, n, x0, y0, z0);
+  int buf = 0;
+  for (;;) {
+    // this tile's halo has landed, and every wave is done with the other
+    // buffer (the previous tile's epilogue)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int k2 = k + stride, t2 = xcd * perXcd + k2;
+    const bool next = k2 < perXcd && t2 < nTiles;
+    int n2 = 0, x2 = 0, y2 = 0, z2 = 0;
+    if (next) {
+      coords(t2, n2, x2, y2, z2);
+      stage16_dma<MT>(a, lds + (buf ^ 1) * BUF, n2, x2, y2, z2);
+    }
+    f32x4 acc[MT][NT];
 #pragma unroll
     for (int r = 0; r < MT; ++r)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float val = acc[r][t][i] + bias;
-        if (a.relu)
-          val = fmaxf(val, 0.0f);
-        ob[(r * 16 + 4 * kg + i) * COUT + co] = f2bf(val);
-      }
-  }
-  __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): this wave's LDS writes are done
-  __builtin_amdgcn_wave_barrier();
-  constexpr int UNITS = MT * 16 * COUT / 8;  // 16-byte units of this wave
-  if (a.y) {
-#pragma unroll
-    for (int u0 = 0; u0 < UNITS; u0 += 64) {
-      const int u = u0 + lane;
-      if (UNITS % 64 == 0 || u < UNITS) {
-        const int c8 = u % (COUT / 8), vx = (u / (COUT / 8)) % 16, r = u / (COUT / 8 * 16);
-        uint4 val = *reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * COUT + 8 * c8);
-        const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
-        uint4 *dst = reinterpret_cast<uint4 *>(a.y + vo * a.ldy + a.yofs + 8 * c8);
-        if (a.accumulate) {
-          const uint4 old = *dst;
-          const uint32_t nv[4] = { val.x, val.y, val.z, val.w };
-          const uint32_t ov[4] = { old.x, old.y, old.z, old.w };
-          uint32_t rv[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            rv[k] = (uint32_t)f2bf(__uint_as_float(nv[k] << 16) + __uint_as_float(ov[k] << 16)) |
-                    ((uint32_t)f2bf(__uint_as_float(nv[k] & 0xffff0000u) +
-                                    __uint_as_float(ov[k] & 0xffff0000u)) << 16);
-          val = make_uint4(rv[0], rv[1], rv[2], rv[3]);
-        }
-        *dst = val;
-      }
-    }
-  }
-  if (a.pool) {
-    // the tile (16 x 4 x MT, all even) holds whole 2x2x2 cells; wave w's rows
-    // are at y0 + w, so a cell reads the buffers of waves 2Y and 2Y + 1
-    __syncthreads();
-    constexpr int C8 = COUT / 8;
-    constexpr int CELLS = 8 * 2 * (MT / 2);     // x cells, y cells, z cells
-    const int Dh = D >> 1;
-    for (int u = threadIdx.x; u < CELLS * C8; u += 256) {
-      const int c8 = u % C8, cell = u / C8;
-      const int cx = cell % 8, cy = (cell / 8) % 2, cz = cell / 16;
-      float sum[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int zz = 2 * cz + (q >> 2), yy = 2 * cy + ((q >> 1) & 1), xx = 2 * cx + (q & 1);
-        const uint4 v = *reinterpret_cast<const uint4 *>(lds + (size_t)yy * MT * 16 * COUT +
-                                                        (zz * 16 + xx) * COUT + 8 * c8);
-        const uint32_t w4[4] = { v.x, v.y, v.z, v.w };
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          sum[2 * k] += __uint_as_float(w4[k] << 16);
-          sum[2 * k + 1] += __uint_as_float(w4[k] & 0xffff0000u);
-        }
-      }
-      uint32_t o[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        o[k] = (uint32_t)f2bf(0.125f * sum[2 * k]) | ((uint32_t)f2bf(0.125f * sum[2 * k + 1]) << 16);
-      const size_t vo = ((((size_t)n * Dh + (z0 >> 1) + cz) * Dh + (y0 >> 1) + cy) * Dh) +
-                        (x0 >> 1) + cx;
-      *reinterpret_cast<uint4 *>(a.pool + vo * a.ldp + a.pofs + 8 * c8) = make_uint4(o[0], o[1], o[2], o[3]);
-    }
-  }
-  if constexpr (COUT == 16) {
-    if (a.hout) {
-      // one voxel per lane: its 16 channels (bfloat16) dotted with the head
-#pragma unroll
-      for (int v0 = 0; v0 < MT * 16; v0 += 64) {
-        const int vv = v0 + lane;
-        if ((MT * 16) % 64 == 0 || vv < MT * 16) {
-          const int vx = vv % 16, r = vv / 16;
-          const uint4 *hv = reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * 16);
-          const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
-          float sum = a.hin[vo] + a.hb[0];
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const uint4 hq = hv[q];
-            const uint32_t hw4[4] = { hq.x, hq.y, hq.z, hq.w };
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              sum = fmaf(a.hw[8 * q + 2 * k], __uint_as_float(hw4[k] << 16), sum);
-              sum = fmaf(a.hw[8 * q + 2 * k + 1], __uint_as_float(hw4[k] & 0xffff0000u), sum);
-            }
-          }
-          a.hout[vo] = sum;
-        }
-      }
-    }
+      for (int tt = 0; tt < NT; ++tt)
+        acc[r][tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    tail16_mfma<NT, MT>(acc, lds + buf * BUF, wrow, wtile, 0, w, m, kg);
+    conv_epilogue<NT, MT>(a, acc, lds + buf * BUF, n, x0, y0, z0, w, lane, m, kg);
+    if (!next)
+      break;
+    k = k2;
+    n = n2, x0 = x2, y0 = y2, z0 = z2;
+    buf ^= 1;
   }
 }
 
@@ -1027,8 +1191,45 @@ template <int NT, int MT, bool TAIL, bool UPC = false> int launch_lds(hipStream_
   return LH_OK;
 }
 
+// z tile of the 16 -> 16 channel convolutions (full resolution, halo-bound)
+#ifndef LH_MT16
+#define LH_MT16 8
+#endif
+
+// Persistent launch of the 16-input-channel convolutions: two workgroups per
+// CU (two halo buffers each), every CU of the device.
+#ifndef LH_CONV16_PERSIST
+#define LH_CONV16_PERSIST 1
+#endif
+template <int NT, int MT> int launch_conv16(hipStream_t s, ConvLdsArgs &a)
+{
+  constexpr size_t lds = (size_t)2 * conv16_buf_elems<NT, MT>() * sizeof(uint16_t);
+  static int grid = 0;
+  if (grid == 0) {
+    UN_HIP(hipFuncSetAttribute((const void *)conv16_persist_kernel<NT, MT>,
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int dev = 0, cus = 0, perCU = 0;
+    UN_HIP(hipGetDevice(&dev));
+    UN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    UN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, conv16_persist_kernel<NT, MT>,
+        256, lds));
+    grid = cus * std::max(perCU, 1);
+  }
+  if (a.D % MT)
+    return lh_fail(LH_E_ARG, "conv: D=%d is not a multiple of the z tile %d", a.D, MT);
+  const long tiles = (long)a.n * (a.D / 16) * (a.D / 4) * (a.D / MT);
+  conv16_persist_kernel<NT, MT><<<(unsigned)std::min<long>(grid, tiles), 256, lds, s>>>(a);
+  return LH_OK;
+}
+
 template <bool TAIL> int launch_cout(hipStream_t s, ConvLdsArgs &a, int cout, bool upc)
 {
+  if (LH_CONV16_PERSIST && TAIL && a.cin == 16 && a.c0 == 16 && !a.s0.half && !upc) {
+    if (cout == 16)
+      return launch_conv16<1, LH_MT16>(s, a);
+    if (cout == 32)
+      return launch_conv16<2, 8>(s, a);
+  }
   if (upc && cout == 16)
     return launch_lds<1, 8, TAIL, true>(s, a);
   if (upc && cout == 32)
