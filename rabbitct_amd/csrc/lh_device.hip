@@ -275,6 +275,9 @@ constexpr int kFence = PREC == LH_PREC_F32 ? LH_FENCE32 : 4;
 // loads at a time is fastest, with R=32 four)
 template <int R>
 constexpr int kFenceL2 = R == 32 ? 4 : 1;
+// voxels per parking segment of a run (the line ranges' alignment)
+template <int R>
+constexpr int kSeg = R < 16 ? R : 16;
 #ifndef LH_OCC16
 #define LH_OCC16 4
 #endif
@@ -428,23 +431,32 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
 // never read; no lane writes past the end of the window; past the end of the
 // slot the buffer bounds return zero.  Completion is awaited by the caller
 // (s_waitcnt vmcnt(0) and a barrier).
+// Lanes whose 16 bytes lie at or beyond the footprint width fw in their row
+// are masked off, so the copies fetch about fw, not PITCH, pixels per row.
 template <class SH>
 struct ChunkOffsets {
   static_assert(!SH::POW2 || 1024 % SH::PITCH == 0, "whole rows per 4 chunks");
+  static_assert(SH::PITCH <= 255, "a column fits 8 bits");
   uint32_t rel[SH::POW2 ? 1 : SH::KMAX];
+  uint32_t cols[SH::POW2 ? 1 : (SH::KMAX + 3) / 4]; // column of chunk k: byte k % 4 of word k / 4
   uint32_t lane0; // 128-dword pitch: rel(0)
   uint32_t step;  // 128-dword pitch: rel(k+1) - rel(k)
   __device__ __forceinline__ void init(int wv, int lane, uint32_t rowBytes)
   {
     if constexpr (SH::POW2) {
       const uint32_t d = (uint32_t)(256 * wv + 4 * lane);
-      lane0 = (d >> SH::LOG2) * rowBytes + (d & (SH::PITCH - 1)) * 4u;
-      step  = (uint32_t)(1024 / SH::PITCH) * rowBytes;
+      lane0   = (d >> SH::LOG2) * rowBytes + (d & (SH::PITCH - 1)) * 4u;
+      step    = (uint32_t)(1024 / SH::PITCH) * rowBytes;
+      cols[0] = d & (SH::PITCH - 1); // the same in every chunk
     } else {
+#pragma unroll
+      for (int k = 0; k < (SH::KMAX + 3) / 4; ++k)
+        cols[k] = 0;
 #pragma unroll
       for (int k = 0; k < SH::KMAX; ++k) {
         const uint32_t d = (uint32_t)(256 * (wv + kWaves * k) + 4 * lane);
         rel[k] = (d / SH::PITCH) * rowBytes + (d % SH::PITCH) * 4u;
+        cols[k / 4] |= (d % SH::PITCH) << (8 * (k % 4));
       }
     }
   }
@@ -454,6 +466,13 @@ struct ChunkOffsets {
       return lane0 + (uint32_t)k * step;
     else
       return rel[k];
+  }
+  __device__ __forceinline__ int col(int k) const
+  {
+    if constexpr (SH::POW2)
+      return (int)cols[0];
+    else
+      return (int)__builtin_amdgcn_ubfe(cols[k / 4], 8 * (k % 4), 8);
   }
 };
 
@@ -471,16 +490,18 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
     // window's last row too when FH is odd).  Measured 2.5% faster per step
     // than a loop over chunk counts at L=512.
     static_assert(SH::PITCH == 128, "row pairs");
-    uint32_t off = base + co.lane0;
-    int r0       = 2 * wv;
-    for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += co.step)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0, 0,
-          0);
-    if (r0 < b.fh && lane < 32)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0, 0,
-          0);
+    if (co.col(0) < b.fw) {
+      uint32_t off = base + co.lane0;
+      int r0       = 2 * wv;
+      for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += co.step)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0,
+            0, 0);
+      if (r0 < b.fh && lane < 32)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0,
+            0, 0);
+    }
   } else {
     // whole chunks only (FH keeps need <= CHUNKS); the opaque copy of the
     // wave index keeps the per-chunk bounds and LDS offsets from being
@@ -493,9 +514,10 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
       const int ch = w + kWaves * k;
       if (ch >= need)
         break;
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-          (__attribute__((address_space(3))) void *)(buf + 256 * ch), 16,
-          (int)(base + co.at(k)), 0, 0, 0);
+      if (co.col(k) < b.fw)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(buf + 256 * ch), 16,
+            (int)(base + co.at(k)), 0, 0, 0);
     }
   }
 }
@@ -655,20 +677,23 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       float u0 = (float)fma(-(double)box.iu0, W0, U0);
       float v0 = (float)fma(-(double)box.iv0, W0, V0);
       float w0 = (float)W0;
+      // dw in a vector register (the empty asm): a voxel's w step r*dw + w'
+      // is then one v_fmamk_f32 with r as its literal, where an SGPR dw would
+      // need r in a VGPR of its own for each r beyond the inline constants
+      float dw = P.dw;
+      __asm__("" : "+v"(dw));
       // the x-steps of u' and v': a*R_L - iu0 * a[2]*R_L
-      const float dw = P.dw;
       const float du = fmaf(-(float)box.iu0, dw, P.du), dv = fmaf(-(float)box.iv0, dw, P.dv);
       // Voxels outside their line's range are "parked": (u', v', w) =
-      // (1/2, 1/2, 1) * 2^100 puts the coordinate half a pixel inside the box
-      // and the weight (1/w)^2 underflows to +0, so the update adds exactly
-      // zero.  The accumulators then take one straight path through every
-      // branch below.
-      const bool full = __all(!active || (rlo == 0 && rhi == R));
-      if (full && !__all(active) && !active) {
-        u0 = kParkUV;
-        v0 = kParkUV;
-        w0 = kPark;
-      }
+      // (1/2, 1/2, 1) * 2^100 with a zero step puts the coordinate half a
+      // pixel inside the box and the weight (1/w)^2 underflows to +0, so the
+      // update adds exactly zero and the accumulators take one straight path
+      // through every branch below.  Ranges aligned to 16 voxels (the clip
+      // files' VECTORSIZE 16) leave every 16-voxel segment of a run all in or
+      // all out: such waves park whole segments (six selects per segment) on
+      // the interior and clamp paths; other waves take the edge path, which
+      // parks voxel by voxel.
+      const bool aligned = __all(!active || ((rlo % kSeg<R>) == 0 && (rhi % kSeg<R>) == 0));
       // Each update loop starts from its own opaque copy of the run origin
       // (an empty asm naming the loop): the loops then share no leading
       // instructions, which the compiler would otherwise hoist (with their
@@ -678,6 +703,15 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
   float uo = u0, vo = v0, wo = w0;                                                       \
   __asm__("; " tag : "+v"(uo), "+v"(vo), "+v"(wo));                                      \
   const float hiu = (float)(box.fw - 2), hiv = (float)(box.fh - 2)
+      // origin and steps of segment h (voxels [h S, h S + S), S = kSeg<R>):
+      // the run's own, or parked with zero steps where the segment lies
+      // outside the lane's range; a voxel's coordinate is fma(r, step,
+      // origin) either way, so in-range voxels get the bits of the unsegmented
+      // form
+#define LH_SEGMENT(h)                                                                    \
+  const bool on_ = rlo <= (h) * kSeg<R> && (h + 1) * kSeg<R> <= rhi;                      \
+  const float us = on_ ? uo : kParkUV, vs = on_ ? vo : kParkUV, ws = on_ ? wo : kPark;   \
+  const float dus = on_ ? du : 0.0f, dvs = on_ ? dv : 0.0f, dws = on_ ? dw : 0.0f
       // Boxes whose origin is left of or above pixel 0 (the detector edge)
       // truncate the global coordinate ix' + iu0; the others add 0, which
       // leaves the local coordinate as it is (kEdge arithmetic with a zero
@@ -688,26 +722,33 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         const float *sBuf = sImg + (v & 1) * SH::BUF;
         // groups of kFence updates are fenced so the scheduler keeps at most
         // that many updates' taps in flight per lane
-        if (full && box.interior) {
+        if (aligned && box.interior) {
           LH_LOOP_ORIGIN("lds interior");
-          // with VECTORSIZE-aligned ranges a run is all in or all out
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const float fr = (float)r;
-            acc[r] += lds_update<PREC, kInterior, SH>(sBuf, r ? fmaf(fr, du, uo) : uo,
-                r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
-            if ((r % kFence<PREC>) == kFence<PREC> - 1)
-              __builtin_amdgcn_sched_barrier(0);
+          for (int h = 0; h < R / kSeg<R>; ++h) {
+            LH_SEGMENT(h);
+#pragma unroll
+            for (int r = h * kSeg<R>; r < (h + 1) * kSeg<R>; ++r) {
+              const float fr = (float)r;
+              acc[r] += lds_update<PREC, kInterior, SH>(sBuf, r ? fmaf(fr, dus, us) : us,
+                  r ? fmaf(fr, dvs, vs) : vs, r ? fmaf(fr, dws, ws) : ws, hiu, hiv, 0.0f, 0.0f);
+              if ((r % kFence<PREC>) == kFence<PREC> - 1)
+                __builtin_amdgcn_sched_barrier(0);
+            }
           }
-        } else if (full && !edge) {
+        } else if (aligned && !edge) {
           LH_LOOP_ORIGIN("lds clamp");
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const float fr = (float)r;
-            acc[r] += lds_update<PREC, kClamp, SH>(sBuf, r ? fmaf(fr, du, uo) : uo,
-                r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
-            if ((r % kFence<PREC>) == kFence<PREC> - 1)
-              __builtin_amdgcn_sched_barrier(0);
+          for (int h = 0; h < R / kSeg<R>; ++h) {
+            LH_SEGMENT(h);
+#pragma unroll
+            for (int r = h * kSeg<R>; r < (h + 1) * kSeg<R>; ++r) {
+              const float fr = (float)r;
+              acc[r] += lds_update<PREC, kClamp, SH>(sBuf, r ? fmaf(fr, dus, us) : us,
+                  r ? fmaf(fr, dvs, vs) : vs, r ? fmaf(fr, dws, ws) : ws, hiu, hiv, 0.0f, 0.0f);
+              if ((r % kFence<PREC>) == kFence<PREC> - 1)
+                __builtin_amdgcn_sched_barrier(0);
+            }
           }
         } else {
           LH_LOOP_ORIGIN("lds edge or partial");
@@ -755,6 +796,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         }
       }
 #undef LH_LOOP_ORIGIN
+#undef LH_SEGMENT
     }
     box      = boxNext;
     blockAct = blockActNext;
@@ -1621,10 +1663,12 @@ static TileDims tile_dims(const LhContext *c)
     return {16, 16, kTileY16};
   if (g.runLength == 16 && c->tileY == 64)
     return {64, 4, kTileY64};
-  // A 32-voxel run spanning more than 8 mm along x (L <= 512) has footprints
+  // A staged 32-voxel run spanning 16 mm along x (L = 512) has footprints
   // wider than 128 pixels at some views: the 160-pixel window, 3 workgroups
-  // per CU.  At 8 mm and below (L >= 1024) the 128-pixel window fits them at 4.
-  if (g.runLength == 32 && 32.0f * g.R_L > 8.0f)
+  // per CU.  At 8 mm (L = 1024) the 128-pixel window fits them at 4; at 32 mm
+  // (L = 256) most exceed both windows and take L2 gathers, which run faster
+  // at 4 workgroups per CU, as does the unstaged form at every size.
+  if (g.runLength == 32 && g.staging && 32.0f * g.R_L > 8.0f && 32.0f * g.R_L <= 16.0f)
     return {32, 8, kTileY32W};
   return {32, 8, kTileY32};
 }
