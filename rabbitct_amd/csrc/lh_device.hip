@@ -32,6 +32,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <cstdarg>
 #include <cstddef>
 #include <atomic>
@@ -523,18 +524,16 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
 }
 
 // One voxel-view update from the LDS footprint (LolaOPT.c:145-170) at local
-// homogeneous (u', v', w): returns val / w^2.  sBase[v * PITCH + u] is
-// footprint pixel (u, v) (kEdge: sBase is offset by the box origin, so it is
-// indexed by global pixels).  The coordinate arithmetic is cell_of's, shared
-// with l2_update.
-template <int PREC, int MODE, class SH>
-__device__ __forceinline__ float lds_update(const float *sBase, float u, float v, float w,
-    float hiu, float hiv, float ou, float ov)
+// homogeneous (u', v', w), in two halves: lds_fetch computes the cell and
+// issues the tap reads, lds_finish interpolates and adds val / w^2 to the
+// accumulator, so a run can have the taps of later voxels in flight while it
+// interpolates earlier ones.  sBase[v * PITCH + u] is footprint pixel (u, v)
+// (kEdge: sBase is offset by the box origin, so it is indexed by global
+// pixels).  The coordinate arithmetic is cell_of's, shared with l2_update.
+// LDS address of the upper-left tap of cell (fx, fy) (integer-valued floats).
+template <int MODE, class SH>
+__device__ __forceinline__ const float *lds_tap_address(const float *sBase, float fx, float fy)
 {
-  const float rw = __builtin_amdgcn_rcpf(w);
-  float ax, ay, fx, fy;
-  cell_of<MODE>(u * rw, v * rw, hiu, hiv, ou, ov, ax, ay, fx, fy);
-  const float *q;
   if constexpr (MODE == kInterior) {
     // byte address base + 4 * column + row * 4 * PITCH as a shift-add and a
     // shift-add (power-of-two pitch) or a 24-bit multiply-add: the empty asm
@@ -544,14 +543,71 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
     uint32_t col = (uint32_t)(uintptr_t)(LdsFloat *)sBase + ((uint32_t)(int)fx << 2);
     __asm__("" : "+v"(col));
     if constexpr (SH::POW2)
-      q = (const float *)(LdsFloat *)(uintptr_t)(col + ((uint32_t)(int)fy << (SH::LOG2 + 2)));
+      return (const float *)(LdsFloat *)(uintptr_t)(col +
+          ((uint32_t)(int)fy << (SH::LOG2 + 2)));
     else
-      q = (const float *)(LdsFloat *)(uintptr_t)(
+      return (const float *)(LdsFloat *)(uintptr_t)(
           __umul24((uint32_t)(int)fy, 4u * SH::PITCH) + col);
   } else {
     // fy*PITCH + fx is an exact integer in FP32 (|.| < 2^24): one conversion
-    q = sBase + (int)fmaf(fy, (float)SH::PITCH, fx);
+    return sBase + (int)fmaf(fy, (float)SH::PITCH, fx);
   }
+}
+
+struct LdsTaps {
+  float p0, p1, p2, p3; // FP32: p00, p10, p01, p11; half: pair dwords of rows j, j+1 in p0, p1
+  float ax, ay, w2;     // fractions and the weight (1/w)^2
+};
+
+template <int PREC, int MODE, class SH>
+__device__ __forceinline__ LdsTaps lds_fetch(const float *sBase, float u, float v, float w,
+    float hiu, float hiv, float ou, float ov)
+{
+  const float rw = __builtin_amdgcn_rcpf(w);
+  LdsTaps t;
+  float fx, fy;
+  cell_of<MODE>(u * rw, v * rw, hiu, hiv, ou, ov, t.ax, t.ay, fx, fy);
+  t.w2 = rw * rw;
+  const float *q = lds_tap_address<MODE, SH>(sBase, fx, fy);
+  if constexpr (PREC == LH_PREC_F32) {
+    t.p0 = q[0];
+    t.p1 = q[1];
+    t.p2 = q[SH::PITCH];
+    t.p3 = q[SH::PITCH + 1];
+  } else {
+    // one dword per pixel: p(i,j) and the forward difference to p(i+1,j)
+    t.p0 = q[0];
+    t.p1 = q[SH::PITCH];
+  }
+  return t;
+}
+
+template <int PREC>
+__device__ __forceinline__ void lds_finish(float &acc, const LdsTaps &t)
+{
+  float b, c;
+  if constexpr (PREC == LH_PREC_F32) {
+    b = fmaf(t.ax, t.p1 - t.p0, t.p0);
+    c = fmaf(t.ax, t.p3 - t.p2, t.p2);
+  } else {
+    b = lerp_pair<PREC>(__float_as_uint(t.p0), t.ax);
+    c = lerp_pair<PREC>(__float_as_uint(t.p1), t.ax);
+  }
+  acc += fmaf(t.ay, c - b, b) * t.w2;
+}
+
+// The whole update in one expression (returns val / w^2): the form the
+// unpipelined runs use.  Written as lds_finish(lds_fetch(...)), with the
+// weight formed before the tap reads, the 16-voxel FP32 kernel took 127
+// instead of 101 VGPRs and ran 15% slower.
+template <int PREC, int MODE, class SH>
+__device__ __forceinline__ float lds_update(const float *sBase, float u, float v, float w,
+    float hiu, float hiv, float ou, float ov)
+{
+  const float rw = __builtin_amdgcn_rcpf(w);
+  float ax, ay, fx, fy;
+  cell_of<MODE>(u * rw, v * rw, hiu, hiv, ou, ov, ax, ay, fx, fy);
+  const float *q = lds_tap_address<MODE, SH>(sBase, fx, fy);
   float b, t;
   if constexpr (PREC == LH_PREC_F32) {
     const float p00 = q[0], p10 = q[1];
@@ -565,6 +621,72 @@ __device__ __forceinline__ float lds_update(const float *sBase, float u, float v
     t = lerp_pair<PREC>(qq[SH::PITCH], ax);
   }
   return fmaf(ay, t - b, b) * (rw * rw);
+}
+
+// f(std::integral_constant<int, I>) for I in [I0, I1): loop indices that
+// are constants from the start, so arrays indexed by them stay in registers
+template <int I0, int I1, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+  if constexpr (I0 < I1) {
+    f(std::integral_constant<int, I0>{});
+    static_for<I0 + 1, I1>(f);
+  }
+}
+
+// Voxels whose taps are in flight while a voxel is interpolated, by run
+// length: 32-voxel runs (3 workgroups per CU, registers to spare) read the
+// taps of voxel r + 1 while interpolating voxel r (measured at L=512: FP32
+// 45.6 -> 44.8 ms per step, FP16 44.5 -> 43.8, BF16 47.3 -> 46.4; 2 to 4
+// voxels ahead ran 0.5-1% slower than one).  Shorter runs (4 workgroups
+// per CU) keep one scheduling group per kFence voxels: pipelined, the
+// compiler hoisted their coordinate arithmetic and spilled (16-voxel FP32
+// 46.3 -> 50.8 ms per step).
+template <int R>
+constexpr int kAhead = R >= 32 ? 1 : 0;
+
+// Voxels [r0, r0 + N) of a run at coordinates fma(r, d, o) (r = 0: o), from
+// the LDS footprint.  kAhead > 0: the taps of voxel r + kAhead are read while
+// voxel r is interpolated, one scheduling group per voxel, so the LDS latency
+// is covered by the arithmetic of kAhead voxels; otherwise one scheduling
+// group per kFence voxels.  The same operations as lds_update per voxel, in
+// another order: the same bits.
+template <int PREC, int MODE, class SH, int N, int r0, int R>
+__device__ __forceinline__ void lds_run(float (&acc)[R], const float *sBase, float uo, float vo,
+    float wo, float du, float dv, float dw, float hiu, float hiv)
+{
+  if constexpr (kAhead<R> == 0) {
+#pragma unroll
+    for (int r = r0; r < r0 + N; ++r) {
+      const float fr = (float)r;
+      acc[r] += lds_update<PREC, MODE, SH>(sBase, r ? fmaf(fr, du, uo) : uo,
+          r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
+      if ((r % kFence<PREC>) == kFence<PREC> - 1)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+  constexpr int D = kAhead<R> < N ? kAhead<R> : N;
+  auto fetch = [&](int r) {
+    const float fr = (float)r;
+    return lds_fetch<PREC, MODE, SH>(sBase, r ? fmaf(fr, du, uo) : uo,
+        r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
+  };
+  LdsTaps t[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    t[k] = fetch(r0 + k);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    LdsTaps next;
+    if (k + D < N)
+      next = fetch(r0 + k + D);
+    lds_finish<PREC>(acc[r0 + k], t[k % D]);
+    if (k + D < N)
+      t[k % D] = next;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  }
 }
 
 template <int R, int PREC, int STAGE, class SH>
@@ -724,32 +846,20 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
         // that many updates' taps in flight per lane
         if (aligned && box.interior) {
           LH_LOOP_ORIGIN("lds interior");
-#pragma unroll
-          for (int h = 0; h < R / kSeg<R>; ++h) {
+          static_for<0, R / kSeg<R>>([&](auto hc) {
+            constexpr int h = decltype(hc)::value;
             LH_SEGMENT(h);
-#pragma unroll
-            for (int r = h * kSeg<R>; r < (h + 1) * kSeg<R>; ++r) {
-              const float fr = (float)r;
-              acc[r] += lds_update<PREC, kInterior, SH>(sBuf, r ? fmaf(fr, dus, us) : us,
-                  r ? fmaf(fr, dvs, vs) : vs, r ? fmaf(fr, dws, ws) : ws, hiu, hiv, 0.0f, 0.0f);
-              if ((r % kFence<PREC>) == kFence<PREC> - 1)
-                __builtin_amdgcn_sched_barrier(0);
-            }
-          }
+            lds_run<PREC, kInterior, SH, kSeg<R>, h * kSeg<R>>(acc, sBuf, us, vs, ws, dus, dvs, dws,
+                hiu, hiv);
+          });
         } else if (aligned && !edge) {
           LH_LOOP_ORIGIN("lds clamp");
-#pragma unroll
-          for (int h = 0; h < R / kSeg<R>; ++h) {
+          static_for<0, R / kSeg<R>>([&](auto hc) {
+            constexpr int h = decltype(hc)::value;
             LH_SEGMENT(h);
-#pragma unroll
-            for (int r = h * kSeg<R>; r < (h + 1) * kSeg<R>; ++r) {
-              const float fr = (float)r;
-              acc[r] += lds_update<PREC, kClamp, SH>(sBuf, r ? fmaf(fr, dus, us) : us,
-                  r ? fmaf(fr, dvs, vs) : vs, r ? fmaf(fr, dws, ws) : ws, hiu, hiv, 0.0f, 0.0f);
-              if ((r % kFence<PREC>) == kFence<PREC> - 1)
-                __builtin_amdgcn_sched_barrier(0);
-            }
-          }
+            lds_run<PREC, kClamp, SH, kSeg<R>, h * kSeg<R>>(acc, sBuf, us, vs, ws, dus, dvs, dws,
+                hiu, hiv);
+          });
         } else {
           LH_LOOP_ORIGIN("lds edge or partial");
           // detector-edge boxes and partial runs, with
