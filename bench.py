@@ -16,6 +16,10 @@ phantom, generated on the GPU).  Reported:
                three streams, timed from the first copy to the last kernel
                (the per-view copy LolaOPT's timed iterate makes,
                LolaOPT.c:111-117), with the copies alone timed beside it
+  h2d_inclusive_half  the same pipeline for the FP16 and BF16 variants with
+               the views held in half precision on the host (converted by
+               lhHostHalfViews, outside the timed region), so half the bytes
+               cross PCIe
   roofline     the headline kernel against its binding ceiling, VALU issue:
                VALU lane-instructions per launch (rocprofv3 SQ_INSTS_VALU,
                profiles/valu.json) / mean launch time (HIP events on the
@@ -184,6 +188,70 @@ def host_pipeline_at(args, geom, ranges, pinned, shape, prec, resident_vol):
     log(f"host pipeline ({prec}, {vpp} views per pass): {ms:.2f} ms/step = {out['gups']} GUPS; H2D alone "
         f"{ms_h2d:.2f} ms ({out['h2d_gb_per_s']} GB/s); max diff vs resident {diff:.2e} of range")
     return out
+
+
+def host_pipeline_half(args, geom, ranges, pinned, shape, prec, resident_vol):
+    """The host-memory pipeline with the views held in half precision on the
+    host (lhHostHalfViews, timed apart as host_convert_ms: a loader would
+    convert while reading the file) and uploaded in 2 bytes a pixel
+    (lhTimedReconstructHostHalf), at the variant's tuned views-per-pass and
+    half of it; the best of the two is reported, with the volume's distance
+    from the resident path of the same precision (the slots differ in the
+    rounding of the forward differences, taken from the rounded halves)."""
+    N = geom.n_views
+    ids = np.arange(1, N + 1, dtype=np.int32)
+    t0 = time.perf_counter()
+    half_pin = lolahip.PinnedArray(pinned.shape, np.uint16)
+    halves, exps = lolahip.host_half_views(prec, pinned, out=half_pin.array)
+    convert_ms = (time.perf_counter() - t0) * 1e3
+    best, tried = None, {}
+    try:
+        for vpp in sorted({shape["views_per_pass"], max(8, shape["views_per_pass"] // 2)},
+                          reverse=True):
+            run, staging = shape["run_length"], shape["staging"]
+            with lolahip.Context(args.L, geom.width, geom.height, N, n_slots=2 * vpp,
+                                 precision=prec, kernel="fused", views_per_pass=vpp,
+                                 run_length=run, staging=staging) as ctx:
+                if args.slab:
+                    ctx.set_slab(*args.slab)
+                ctx.set_tile(shape.get("tile_y", 32))
+                ctx.set_ranges(ranges)
+                ctx.set_graphs(True)
+                ctx.reconstruct_host_half(halves, exps, geom.matrices, ids)
+                ctx.zero_volume()
+                ctx.reconstruct_host_half(halves, exps, geom.matrices, ids)
+                vol = ctx.get_volume()
+                rng = float(resident_vol.max() - resident_vol.min())
+                diff = float(np.abs(vol - resident_vol).max()) / rng
+                del vol
+                if args.warmup:
+                    ctx.timed_reconstruct_host_half(halves, exps, geom.matrices, ids, args.warmup)
+                ms = ctx.timed_reconstruct_host_half(halves, exps, geom.matrices, ids, args.steps)
+                ms_h2d = ctx.timed_upload_host_half(halves, max(1, min(args.steps, 3)))
+            tried[vpp] = round(ms, 3)
+            depth = args.slab[1] - args.slab[0] if args.slab else args.L
+            r = {"ms_per_step": round(ms, 3),
+                 "gups": round(depth * args.L ** 2 * N / ms / 1e6, 2),
+                 "h2d_only_ms": round(ms_h2d, 3),
+                 "h2d_gb_per_s": round(halves.nbytes / ms_h2d / 1e6, 2),
+                 "h2d_bytes_per_step": int(halves.nbytes),
+                 "max_diff_vs_resident": diff,
+                 "shape": {**shape, "views_per_pass": vpp, "slots": 2 * vpp, "graphs": True}}
+            log(f"half host pipeline ({prec}, {vpp} views per pass): {ms:.2f} ms/step = "
+                f"{r['gups']} GUPS; H2D alone {ms_h2d:.2f} ms; max diff vs resident "
+                f"{diff:.2e} of range")
+            if best is None or ms < best["ms_per_step"]:
+                best = r
+    finally:
+        half_pin.close()
+    best["ms_per_step_by_views_per_pass"] = tried
+    best["host_convert_ms"] = round(convert_ms, 1)
+    best["how"] = ("lhTimedReconstructHostHalf: views converted to " + prec + " on the host "
+                   "(lhHostHalfViews, per-view power-of-4 scale for FP16; not timed: "
+                   "host_convert_ms) in pinned memory, 2-byte H2D copies, pairs formed by "
+                   "the staging launch, graph-captured fused batches; HIP events from the "
+                   "first copy to the last kernel")
+    return best
 
 
 def cpu_model():
@@ -630,6 +698,13 @@ def main():
     if pinned is not None and args.headline.startswith("fused"):
         h2d = host_pipeline(args, geom, ranges, pinned.array, head["shape"], h_prec,
                             vols[args.headline])
+    h2d_half = {}
+    if pinned is not None and args.headline.startswith("fused"):
+        for var in variants:
+            kernel, prec = var.split("-")
+            if kernel == "fused" and prec in ("f16", "bf16") and var in vols:
+                h2d_half[prec] = host_pipeline_half(args, geom, ranges, pinned.array,
+                                                    results[var]["shape"], prec, vols[var])
     if pinned is not None:
         pinned.close()
     line = {
@@ -656,6 +731,7 @@ def main():
         "gups_effective": head["gups_effective"],
         "variants": results,
         "h2d_inclusive": h2d,
+        "h2d_inclusive_half": h2d_half or None,
         "roofline": roofline(args, None, head, lat[args.headline], slot_bytes),
         "gather_roof": gather_roof(head),
         "cpu_baseline": None,

@@ -252,6 +252,27 @@ int lhHostFree(void *p);
 int lhReconstructHost(LhContext *ctx, const float *images, const double *matrices,
     const int *ids, uint32_t count);
 
+/* Half-precision host copies of views for lhReconstructHostHalf: count views
+ * of W x H floats (images) become count x W x H halves (FP16 or BF16 bit
+ * patterns, precision LH_PREC_F16 / LH_PREC_BF16) and one storage exponent m
+ * per view: FP16 holds p * 4^-m with m such that max|p| 4^-m lies in
+ * (4096, 16384], BF16 holds p (m = 0); both round to nearest even, as the
+ * device staging of float uploads rounds.  Host code (all host threads).  No
+ * counterpart in the reference, whose projections cross into the module as
+ * float32 (rabbitCt.h:20-27). */
+int lhHostHalfViews(int precision, const float *images, uint32_t count, uint32_t W, uint32_t H,
+    uint16_t *halves, int32_t *exponents);
+
+/* lhReconstructHost from half-precision host views (lhHostHalfViews output:
+ * count x H x W halves and their storage exponents, for the context's
+ * precision, BF16 or FP16): half the bytes cross PCIe.  The staging forms
+ * each slot's pairs from the uploaded halves: h(i,j) and the forward
+ * difference h(i+1,j) - h(i,j) rounded to the storage precision (float uploads
+ * take the difference from the unrounded p(i+1,j)).  No counterpart in the
+ * reference. */
+int lhReconstructHostHalf(LhContext *ctx, const uint16_t *halves, const int32_t *exponents,
+    const double *matrices, const int *ids, uint32_t count);
+
 /* The batch sizes lhReconstructHost uses for `count` views in batches of
  * `batch` (the first and the last two shortened): returns the number of
  * batches and writes up to maxSizes of them to sizes (may be null). */
@@ -263,12 +284,20 @@ uint32_t lhHostBatchPlan(uint32_t count, uint32_t batch, uint32_t *sizes, uint32
 int lhTimedReconstructHost(LhContext *ctx, const float *images, const double *matrices,
     const int *ids, uint32_t count, int reps, float *msPerRep);
 
+/* lhTimedReconstructHost for lhReconstructHostHalf. */
+int lhTimedReconstructHostHalf(LhContext *ctx, const uint16_t *halves, const int32_t *exponents,
+    const double *matrices, const int *ids, uint32_t count, int reps, float *msPerRep);
+
 /* The PCIe leg of lhReconstructHost alone: the H2D copies of views
  * [0, count) of images (host, count x H x W floats) into its device ring, one
  * copy per batch as lhReconstructHost makes them, back to back on the upload
  * stream, `reps` times; mean milliseconds per repetition (HIP events).  No
  * counterpart in the reference. */
 int lhTimedUploadHost(LhContext *ctx, const float *images, uint32_t count, int reps,
+    float *msPerRep);
+
+/* lhTimedUploadHost for half-precision views (count x H x W halves). */
+int lhTimedUploadHostHalf(LhContext *ctx, const uint16_t *halves, uint32_t count, int reps,
     float *msPerRep);
 
 /* enable != 0: lhBackproject replays one captured HIP graph per batch (slot

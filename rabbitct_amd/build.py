@@ -7,6 +7,7 @@
   lh_metric.hip  -> hipcc (the -c HU metric on the GPU)
   lh_roof.hip    -> hipcc (gather roof microbenchmark)
   lh_module.c    -> gcc -fopenmp (RabbitCT module, clip files, CPU ranges)
+  lh_hosthalf.c  -> gcc -fopenmp (half-precision host views)
   link           -> hipcc -shared, in-tree: rabbitct_amd/liblolahip.so
 """
 from __future__ import annotations
@@ -29,7 +30,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-I" + INCLU
              "-munsafe-fp-atomics", "-Wall"]
 HIP_SOURCES = ["lh_device.hip", "lh_phantom.hip", "lh_unet.hip", "lh_learned.hip",
                "lh_metric.hip", "lh_roof.hip"]
-C_SOURCES = ["lh_module.c"]
+C_SOURCES = ["lh_module.c", "lh_hosthalf.c"]
 C_FLAGS = ["-O3", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wextra", "-I" + INCLUDE]
 
 

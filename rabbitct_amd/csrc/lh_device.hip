@@ -1097,6 +1097,36 @@ __global__ void stage_batch_kernel(const float *__restrict__ img, uint64_t imgSt
       blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0);
 }
 
+// A batch of views uploaded in half precision (lhReconstructHostHalf): view
+// z = blockIdx.z, W x H halves at img + z * imgStride, already scaled by the
+// host (lhHostHalfViews), into the pair layout of slot z: element (i,j)
+// holds h(i,j) and the forward difference h(i+1,j) - h(i,j) of the uploaded
+// halves, rounded to the storage precision (zero outside the image).  The
+// view parameters vpIn[z] carry the scale already.
+template <int PREC>
+__global__ void stage_half_batch_kernel(const uint16_t *__restrict__ img, uint64_t imgStride,
+    int W, int H, uint8_t *slots, uint64_t slotBytes, uint32_t rowBytes,
+    const ViewParams *__restrict__ vpIn, ViewParams *out)
+{
+  const int z  = blockIdx.z;
+  const int ei = blockIdx.x * blockDim.x + threadIdx.x, ej = blockIdx.y;
+  img += (uint64_t)z * imgStride;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+    out[z] = vpIn[z];
+  if (ei >= W + 3)
+    return;
+  const int i = ei - (int)kBorder, j = ej - (int)kBorder;
+  auto hpix = [&](int ii) -> uint32_t {
+    return (ii >= 0 && ii < W && j >= 0 && j < H) ? img[(size_t)j * W + ii] : 0u;
+  };
+  const uint32_t h0 = hpix(i), h1 = hpix(i + 1);
+  const float p0 = PREC == LH_PREC_BF16 ? bf16_lo(h0) : f16_lo(h0);
+  const float p1 = PREC == LH_PREC_BF16 ? bf16_lo(h1) : f16_lo(h1);
+  const uint16_t d = PREC == LH_PREC_BF16 ? to_bf16(p1 - p0) : to_f16(p1 - p0);
+  reinterpret_cast<uint32_t *>(slots + (uint64_t)z * slotBytes + (size_t)ej * rowBytes)[ei] =
+      h0 | ((uint32_t)d << 16);
+}
+
 // vol *= s (float4 body, scalar tail)
 __global__ void scale_kernel(float *__restrict__ vol, uint64_t n, float s)
 {
@@ -2043,6 +2073,8 @@ static std::vector<uint32_t> host_batches(uint32_t count, uint32_t batch)
 // parameter buffers, kept for later calls of the same batch size.
 static int host_ring(LhContext *c, uint32_t batch)
 {
+  // sized for float images; half-precision uploads use the first half of
+  // each half of the ring
   if (c->hostBatch == batch)
     return LH_OK;
   const size_t px = (size_t)c->cfg.W * c->cfg.H;
@@ -2075,19 +2107,19 @@ uint32_t lhHostBatchPlan(uint32_t count, uint32_t batch, uint32_t *sizes, uint32
   return (uint32_t)p.size();
 }
 
-int lhReconstructHost(LhContext *c, const float *images, const double *matrices, const int *ids,
-    uint32_t count)
+// lhReconstructHost (halves == nullptr: float images) and
+// lhReconstructHostHalf (halves and their storage exponents).
+static int reconstruct_host(LhContext *c, const float *images, const uint16_t *halves,
+    const int32_t *exps, const double *matrices, const int *ids, uint32_t count)
 {
-  if (!c || !images || !matrices || !ids || count == 0)
-    return fail(LH_E_ARG, "lhReconstructHost: bad arguments");
   const LhConfig &g    = c->cfg;
+  const bool half      = halves != nullptr;
+  const size_t elem    = half ? sizeof(uint16_t) : sizeof(float);
+  const uint8_t *src   = half ? (const uint8_t *)halves : (const uint8_t *)images;
   const uint32_t batch = std::min(g.nSlots,
       g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 16u);
   const uint32_t nsets = g.nSlots / batch;
-  for (uint32_t v = 0; v < count; ++v)
-    if (ids[v] < 1 || (uint32_t)ids[v] > g.nViews)
-      return fail(LH_E_ARG, "lhReconstructHost: view id %d out of range", ids[v]);
-  const size_t px = (size_t)g.W * g.H;
+  const size_t px      = (size_t)g.W * g.H;
   LH_HIP(hipSetDevice(g.device));
   {
     const int rc = host_ring(c, batch);
@@ -2107,8 +2139,21 @@ int lhReconstructHost(LhContext *c, const float *images, const double *matrices,
   } else {
     LH_HIP(hipStreamSynchronize(c->upStream)); // the buffer is rewritten below
   }
-  for (uint32_t v = 0; v < count; ++v)
-    c->hHostParams[v] = view_params_of(g, ids[v], matrices + 12 * (size_t)v);
+  for (uint32_t v = 0; v < count; ++v) {
+    ViewParams &vp = c->hHostParams[v];
+    vp             = view_params_of(g, ids[v], matrices + 12 * (size_t)v);
+    if (half && exps[v] != 0) {
+      // the storage scale 4^m folded into the matrix exactly, as the
+      // staging of float uploads does (stage_element)
+      const double k = std::ldexp(1.0, -exps[v]);
+      for (int q = 0; q < 12; ++q)
+        vp.a[q] *= k;
+      vp.du *= (float)k;
+      vp.dv *= (float)k;
+      vp.dw *= (float)k;
+      vp.scale = std::ldexp(1.0f, 2 * exps[v]);
+    }
+  }
   // the first copies wait for the compute stream's earlier work (a previous
   // call's staging may still read the ring)
   LH_HIP(hipEventRecord(c->halfFree[0], c->stream));
@@ -2117,22 +2162,33 @@ int lhReconstructHost(LhContext *c, const float *images, const double *matrices,
   const int cols = g.precision == LH_PREC_F32 ? (int)g.W + 4 : (int)g.W + 3;
   uint32_t v0 = 0;
   for (size_t k = 0; k < sizes.size(); ++k) {
-    const uint32_t n = sizes[k], half = (uint32_t)(k & 1);
+    const uint32_t n = sizes[k], hr = (uint32_t)(k & 1);
     const uint32_t base = (uint32_t)(k % nsets) * batch;
-    float *ring         = c->dHostRing + (size_t)half * batch * px;
-    ViewParams *params  = c->dHostParams + (size_t)half * batch;
-    // upload stream: the half's previous staging has read it
-    LH_HIP(hipStreamWaitEvent(c->upStream, c->halfFree[half], 0));
+    float *ring         = c->dHostRing + (size_t)hr * batch * px;
+    ViewParams *params  = c->dHostParams + (size_t)hr * batch;
+    // upload stream: the ring half's previous staging has read it
+    LH_HIP(hipStreamWaitEvent(c->upStream, c->halfFree[hr], 0));
     // the batch's images are contiguous on both sides: one copy
-    LH_HIP(hipMemcpyAsync(ring, images + (size_t)v0 * px, (size_t)n * px * sizeof(float),
+    LH_HIP(hipMemcpyAsync(ring, src + (size_t)v0 * px * elem, (size_t)n * px * elem,
         hipMemcpyHostToDevice, c->upStream));
     LH_HIP(hipMemcpyAsync(params, c->hHostParams + v0, n * sizeof(ViewParams),
         hipMemcpyHostToDevice, c->upStream));
-    LH_HIP(hipEventRecord(c->halfCopied[half], c->upStream));
+    LH_HIP(hipEventRecord(c->halfCopied[hr], c->upStream));
     // compute stream: stage the batch (stream order puts it after the
     // backprojection that last read these slots), then backproject it
-    LH_HIP(hipStreamWaitEvent(c->stream, c->halfCopied[half], 0));
-    if (g.precision == LH_PREC_F16) {
+    LH_HIP(hipStreamWaitEvent(c->stream, c->halfCopied[hr], 0));
+    uint8_t *slots = c->dSlots + (uint64_t)base * c->slotBytes;
+    if (half) {
+      const dim3 grid(((int)g.W + 3 + 255) / 256, g.H + 4, n);
+      const uint16_t *ring16 = reinterpret_cast<const uint16_t *>(ring);
+      if (g.precision == LH_PREC_BF16)
+        stage_half_batch_kernel<LH_PREC_BF16><<<grid, 256, 0, c->stream>>>(ring16, px, g.W, g.H,
+            slots, c->slotBytes, c->rowBytes, params, c->dViews + base);
+      else
+        stage_half_batch_kernel<LH_PREC_F16><<<grid, 256, 0, c->stream>>>(ring16, px, g.W, g.H,
+            slots, c->slotBytes, c->rowBytes, params, c->dViews + base);
+      LH_HIP(hipGetLastError());
+    } else if (g.precision == LH_PREC_F16) {
       LH_HIP(hipMemsetAsync(c->dMax + base, 0, n * sizeof(uint32_t), c->stream));
       const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(
           (uint32_t)((px / 4 + 4 * kMaxAbsThreads - 1) / (4 * kMaxAbsThreads)), 256u));
@@ -2140,8 +2196,9 @@ int lhReconstructHost(LhContext *c, const float *images, const double *matrices,
           c->dMax + base, px);
     }
     const dim3 grid((cols + 255) / 256, g.H + 4, n);
-    uint8_t *slots = c->dSlots + (uint64_t)base * c->slotBytes;
-    switch (g.precision) {
+    switch (half ? -1 : (int)g.precision) {
+    case -1:
+      break;
     case LH_PREC_F32:
       stage_batch_kernel<LH_PREC_F32><<<grid, 256, 0, c->stream>>>(ring, px, g.W, g.H, slots,
           c->slotBytes, c->rowBytes, c->dMax + base, params, c->dViews + base);
@@ -2156,7 +2213,7 @@ int lhReconstructHost(LhContext *c, const float *images, const double *matrices,
       break;
     }
     LH_HIP(hipGetLastError());
-    LH_HIP(hipEventRecord(c->halfFree[half], c->stream));
+    LH_HIP(hipEventRecord(c->halfFree[hr], c->stream));
     for (uint32_t j = 0; j < n; ++j) {
       c->slotId[base + j] = ids[v0 + j];
       c->boxR[base + j]   = 0;
@@ -2169,17 +2226,59 @@ int lhReconstructHost(LhContext *c, const float *images, const double *matrices,
   return LH_OK;
 }
 
-int lhTimedReconstructHost(LhContext *c, const float *images, const double *matrices,
-    const int *ids, uint32_t count, int reps, float *msPerRep)
+static int host_args_check(LhContext *c, const void *images, const double *matrices,
+    const int *ids, uint32_t count, const char *who)
 {
-  if (!c || reps < 1)
-    return fail(LH_E_ARG, "lhTimedReconstructHost: bad arguments");
+  if (!c || !images || !matrices || !ids || count == 0)
+    return fail(LH_E_ARG, "%s: bad arguments", who);
+  for (uint32_t v = 0; v < count; ++v)
+    if (ids[v] < 1 || (uint32_t)ids[v] > c->cfg.nViews)
+      return fail(LH_E_ARG, "%s: view id %d out of range", who, ids[v]);
+  return LH_OK;
+}
+
+int lhReconstructHost(LhContext *c, const float *images, const double *matrices, const int *ids,
+    uint32_t count)
+{
+  const int rc = host_args_check(c, images, matrices, ids, count, "lhReconstructHost");
+  return rc != LH_OK ? rc : reconstruct_host(c, images, nullptr, nullptr, matrices, ids, count);
+}
+
+static int half_args_check(LhContext *c, const uint16_t *halves, const int32_t *exps,
+    const double *matrices, const int *ids, uint32_t count, const char *who)
+{
+  const int rc = host_args_check(c, halves, matrices, ids, count, who);
+  if (rc != LH_OK)
+    return rc;
+  if (!exps)
+    return fail(LH_E_ARG, "%s: no storage exponents", who);
+  if (c->cfg.precision == LH_PREC_F32)
+    return fail(LH_E_ARG, "%s: the context stores FP32 projections", who);
+  for (uint32_t v = 0; v < count; ++v)
+    if (exps[v] < -60 || exps[v] > 60 || (c->cfg.precision == LH_PREC_BF16 && exps[v] != 0))
+      return fail(LH_E_ARG, "%s: storage exponent %d of view %u out of range", who, exps[v], v);
+  return LH_OK;
+}
+
+int lhReconstructHostHalf(LhContext *c, const uint16_t *halves, const int32_t *exps,
+    const double *matrices, const int *ids, uint32_t count)
+{
+  const int rc = half_args_check(c, halves, exps, matrices, ids, count, "lhReconstructHostHalf");
+  return rc != LH_OK ? rc : reconstruct_host(c, nullptr, halves, exps, matrices, ids, count);
+}
+
+// Repeats of reconstruct_host, each from idle streams, timed with HIP events
+// from before the first H2D copy to after the last backprojection.
+static int timed_reconstruct(LhContext *c, const float *images, const uint16_t *halves,
+    const int32_t *exps, const double *matrices, const int *ids, uint32_t count, int reps,
+    float *msPerRep)
+{
   LH_HIP(hipSetDevice(c->cfg.device));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   LH_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
     (void)hipEventDestroy(e0);
-    return fail(LH_E_HIP, "lhTimedReconstructHost: event creation failed");
+    return fail(LH_E_HIP, "host reconstruction timing: event creation failed");
   }
   double total = 0.0;
   int rc       = LH_OK;
@@ -2191,10 +2290,10 @@ int lhTimedReconstructHost(LhContext *c, const float *images, const double *matr
     if (e == hipSuccess)
       e = hipEventRecord(e0, c->upStream);
     if (e != hipSuccess) {
-      rc = fail(LH_E_HIP, "lhTimedReconstructHost: %s", hipGetErrorString(e));
+      rc = fail(LH_E_HIP, "host reconstruction timing: %s", hipGetErrorString(e));
       break;
     }
-    rc = lhReconstructHost(c, images, matrices, ids, count);
+    rc = reconstruct_host(c, images, halves, exps, matrices, ids, count);
     if (rc != LH_OK)
       break;
     float ms = 0.0f;
@@ -2204,7 +2303,7 @@ int lhTimedReconstructHost(LhContext *c, const float *images, const double *matr
     if (e == hipSuccess)
       e = hipEventElapsedTime(&ms, e0, e1);
     if (e != hipSuccess)
-      rc = fail(LH_E_HIP, "lhTimedReconstructHost: %s", hipGetErrorString(e));
+      rc = fail(LH_E_HIP, "host reconstruction timing: %s", hipGetErrorString(e));
     total += ms;
   }
   (void)hipStreamSynchronize(c->stream);
@@ -2215,11 +2314,32 @@ int lhTimedReconstructHost(LhContext *c, const float *images, const double *matr
   return rc;
 }
 
-int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int reps,
+int lhTimedReconstructHost(LhContext *c, const float *images, const double *matrices,
+    const int *ids, uint32_t count, int reps, float *msPerRep)
+{
+  int rc = host_args_check(c, images, matrices, ids, count, "lhTimedReconstructHost");
+  if (rc == LH_OK && reps < 1)
+    rc = fail(LH_E_ARG, "lhTimedReconstructHost: bad arguments");
+  return rc != LH_OK ? rc
+                     : timed_reconstruct(c, images, nullptr, nullptr, matrices, ids, count, reps,
+                           msPerRep);
+}
+
+int lhTimedReconstructHostHalf(LhContext *c, const uint16_t *halves, const int32_t *exps,
+    const double *matrices, const int *ids, uint32_t count, int reps, float *msPerRep)
+{
+  int rc = half_args_check(c, halves, exps, matrices, ids, count, "lhTimedReconstructHostHalf");
+  if (rc == LH_OK && reps < 1)
+    rc = fail(LH_E_ARG, "lhTimedReconstructHostHalf: bad arguments");
+  return rc != LH_OK ? rc
+                     : timed_reconstruct(c, nullptr, halves, exps, matrices, ids, count, reps,
+                           msPerRep);
+}
+
+// The H2D copies of reconstruct_host alone (elem bytes per pixel).
+static int timed_upload(LhContext *c, const void *images, size_t elem, uint32_t count, int reps,
     float *msPerRep)
 {
-  if (!c || !images || count == 0 || reps < 1)
-    return fail(LH_E_ARG, "lhTimedUploadHost: bad arguments");
   const LhConfig &g    = c->cfg;
   const uint32_t batch = std::min(g.nSlots,
       g.kernel == LH_KERNEL_FUSED ? std::max(1u, g.viewsPerPass) : 16u);
@@ -2234,7 +2354,7 @@ int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int rep
   LH_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
     (void)hipEventDestroy(e0);
-    return fail(LH_E_HIP, "lhTimedUploadHost: event creation failed");
+    return fail(LH_E_HIP, "host upload timing: event creation failed");
   }
   hipError_t e = hipSuccess;
   double total = 0.0;
@@ -2248,8 +2368,9 @@ int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int rep
       e = hipEventRecord(e0, c->upStream);
     uint32_t v0 = 0;
     for (size_t k = 0; k < sizes.size() && e == hipSuccess; v0 += sizes[k], ++k)
-      e = hipMemcpyAsync(c->dHostRing + (k & 1) * (size_t)batch * px, images + (size_t)v0 * px,
-          (size_t)sizes[k] * px * sizeof(float), hipMemcpyHostToDevice, c->upStream);
+      e = hipMemcpyAsync(c->dHostRing + (k & 1) * (size_t)batch * px,
+          (const uint8_t *)images + (size_t)v0 * px * elem, (size_t)sizes[k] * px * elem,
+          hipMemcpyHostToDevice, c->upStream);
     if (e == hipSuccess)
       e = hipEventRecord(e1, c->upStream);
     if (e == hipSuccess)
@@ -2263,10 +2384,26 @@ int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int rep
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (e != hipSuccess)
-    return fail(LH_E_HIP, "lhTimedUploadHost: %s", hipGetErrorString(e));
+    return fail(LH_E_HIP, "host upload timing: %s", hipGetErrorString(e));
   if (msPerRep)
     *msPerRep = (float)(total / reps);
   return LH_OK;
+}
+
+int lhTimedUploadHost(LhContext *c, const float *images, uint32_t count, int reps,
+    float *msPerRep)
+{
+  if (!c || !images || count == 0 || reps < 1)
+    return fail(LH_E_ARG, "lhTimedUploadHost: bad arguments");
+  return timed_upload(c, images, sizeof(float), count, reps, msPerRep);
+}
+
+int lhTimedUploadHostHalf(LhContext *c, const uint16_t *halves, uint32_t count, int reps,
+    float *msPerRep)
+{
+  if (!c || !halves || count == 0 || reps < 1)
+    return fail(LH_E_ARG, "lhTimedUploadHostHalf: bad arguments");
+  return timed_upload(c, halves, sizeof(uint16_t), count, reps, msPerRep);
 }
 
 int lhBackprojectNoAlloc(LhContext *c, uint32_t first, uint32_t count)

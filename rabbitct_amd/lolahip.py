@@ -87,6 +87,9 @@ EXPORTS = (
     # host-memory reconstruction and the module's timing contract
     "lhQueryIdle", "lhHostAlloc", "lhHostFree", "lhReconstructHost", "lhTimedReconstructHost",
     "lolaHipQueryIdle", "lhDeviceInfo", "lhTimedUploadHost",
+    # half-precision host views
+    "lhHostHalfViews", "lhReconstructHostHalf", "lhTimedReconstructHostHalf",
+    "lhTimedUploadHostHalf",
     # allocation-free backprojection
     "lhBackprojectNoAlloc", "lhHostBatchPlan",
 )
@@ -187,6 +190,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lolaHipQueryIdle.argtypes = []
     lib.lhDeviceInfo.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.lhTimedUploadHost.argtypes = [P, P, u32, i32, ctypes.POINTER(f32)]
+    lib.lhHostHalfViews.argtypes = [i32, P, u32, u32, u32, P, P]
+    lib.lhReconstructHostHalf.argtypes = [P, P, P, P, P, u32]
+    lib.lhTimedReconstructHostHalf.argtypes = [P, P, P, P, P, u32, i32, ctypes.POINTER(f32)]
+    lib.lhTimedUploadHostHalf.argtypes = [P, P, u32, i32, ctypes.POINTER(f32)]
     lib.lhBackprojectNoAlloc.argtypes = [P, u32, u32]
     lib.lhHostBatchPlan.argtypes = [u32, u32, P, u32]
     lib.lhHostBatchPlan.restype = u32
@@ -288,6 +295,26 @@ def device_info(device: int = 0) -> dict:
     _check(load_library().lhDeviceInfo(device, ctypes.byref(cu), ctypes.byref(khz)),
            "lhDeviceInfo")
     return {"compute_units": cu.value, "clock_mhz": khz.value / 1000.0}
+
+
+def host_half_views(precision: str, images, out=None):
+    """lhHostHalfViews: (halves, exponents) for float32 views (N,H,W): uint16
+    bit patterns of FP16 (each view scaled by 4^-m) or BF16, and the int32
+    storage exponents m.  ``out``: an (N,H,W) uint16 array to fill (a
+    PinnedArray's array for asynchronous uploads)."""
+    prec = {"f16": 2, "bf16": 1}.get(precision)
+    if prec is None:
+        raise LolaHipError("half views are f16 or bf16")
+    images = np.ascontiguousarray(images, dtype=np.float32)
+    n, h, w = images.shape
+    if out is None:
+        out = np.empty((n, h, w), np.uint16)
+    elif out.shape != images.shape or out.dtype != np.uint16 or not out.flags.c_contiguous:
+        raise LolaHipError("out must be a C-contiguous uint16 array of the images' shape")
+    exps = np.zeros(n, np.int32)
+    _check(load_library().lhHostHalfViews(prec, _p(images), n, w, h, _p(out), _p(exps)),
+           "lhHostHalfViews")
+    return out, exps
 
 
 def host_batch_plan(count: int, batch: int) -> list:
@@ -476,6 +503,47 @@ class Context:
         ms = ctypes.c_float()
         _check(self.lib.lhTimedUploadHost(self._h, _p(images), images.shape[0], reps,
                                           ctypes.byref(ms)), "lhTimedUploadHost")
+        return ms.value
+
+    @staticmethod
+    def _half_args(halves, exps, matrices, ids):
+        if not (isinstance(halves, np.ndarray) and halves.dtype == np.uint16
+                and halves.flags.c_contiguous):
+            raise LolaHipError("halves must be a C-contiguous uint16 array (pinned memory "
+                               "from PinnedArray for asynchronous copies)")
+        e = np.ascontiguousarray(exps, dtype=np.int32)
+        m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1, 12)
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        if not (halves.shape[0] == m.shape[0] == i.shape[0] == e.shape[0]):
+            raise LolaHipError("halves, exponents, matrices and ids disagree in the number "
+                               "of views")
+        return e, m, i
+
+    def reconstruct_host_half(self, halves, exps, matrices, ids):
+        """lhReconstructHostHalf: reconstruct_host from half-precision host views
+        (host_half_views' output, for this context's precision)."""
+        e, m, i = self._half_args(halves, exps, matrices, ids)
+        _check(self.lib.lhReconstructHostHalf(self._h, _p(halves), _p(e), _p(m), _p(i),
+                                              m.shape[0]), "lhReconstructHostHalf")
+        self.synchronize()
+
+    def timed_reconstruct_host_half(self, halves, exps, matrices, ids, reps):
+        """lhTimedReconstructHostHalf: mean ms per repetition."""
+        e, m, i = self._half_args(halves, exps, matrices, ids)
+        ms = ctypes.c_float()
+        _check(self.lib.lhTimedReconstructHostHalf(self._h, _p(halves), _p(e), _p(m), _p(i),
+                                                   m.shape[0], reps, ctypes.byref(ms)),
+               "lhTimedReconstructHostHalf")
+        return ms.value
+
+    def timed_upload_host_half(self, halves, reps):
+        """lhTimedUploadHostHalf: mean ms per repetition of the H2D copies alone."""
+        if not (isinstance(halves, np.ndarray) and halves.dtype == np.uint16
+                and halves.flags.c_contiguous):
+            raise LolaHipError("halves must be a C-contiguous uint16 array")
+        ms = ctypes.c_float()
+        _check(self.lib.lhTimedUploadHostHalf(self._h, _p(halves), halves.shape[0], reps,
+                                              ctypes.byref(ms)), "lhTimedUploadHostHalf")
         return ms.value
 
     def timed_reconstruct_host(self, images, matrices, ids, reps):

@@ -138,3 +138,66 @@ def test_timed_host_pipeline_reports_a_time(headline):
             assert ctx.query_idle()
     print(f"\nhost pipeline, 62 views at L=512: {ms:.2f} ms per run")
     assert ms > 0.0
+
+
+def _dequantized(halves, exps, prec):
+    """The float32 images the half-precision host views stand for: each half
+    widened exactly and the FP16 storage scale 4^m undone (exact)."""
+    if prec == "f16":
+        f = halves.view(np.float16).astype(np.float32)
+        return (f * np.ldexp(np.float32(1.0), 2 * exps)[:, None, None].astype(np.float32))
+    return (halves.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("prec,vpp", [("f16", 16), ("bf16", 64)])
+def test_half_host_pipeline(headline, prec, vpp):
+    """lhReconstructHostHalf: the views converted to half precision on the
+    host (lhHostHalfViews) cross PCIe in 2 bytes a pixel and are staged into
+    the pair layout on the device.  Checked three ways: (1) bit for bit
+    against the resident path fed with the images the halves stand for
+    (where the staging of float uploads rounds nothing, so both form the same
+    pairs and scales; the batches are the pipeline's); (2) the sampled lines
+    against the FP64 oracle of the original views, RMSE <= 1e-3 of range (the
+    stated bar) and a regression bound; (3) the timed form reports a time."""
+    h = headline
+    n = len(h["mats"])
+    H, W = h["imgs"].shape[1:]
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    plan = lolahip.host_batch_plan(n, vpp)
+    print(f"\ncase: half host pipeline {prec} vpp={vpp}, batches {plan}", flush=True)
+    with lolahip.PinnedArray(h["imgs"].shape, np.uint16) as pin:
+        halves, exps = lolahip.host_half_views(prec, h["imgs"], out=pin.array)
+        deq = _dequantized(halves, exps, prec)
+        if prec == "f16":
+            # the float images give the same storage exponents (the max of a
+            # view moves by less than its rounding)
+            _, e2 = lolahip.host_half_views(prec, deq)
+            assert np.array_equal(e2, exps)
+        with lolahip.Context(L, W, H, n, n_slots=2 * vpp, precision=prec, kernel="fused",
+                             views_per_pass=vpp, run_length=16) as ctx:
+            ctx.set_ranges(h["ranges"])
+            ctx.reconstruct_host_half(halves, exps, h["mats"], ids)
+            assert ctx.query_idle()
+            got = ctx.get_volume()
+            ms = ctx.timed_reconstruct_host_half(halves, exps, h["mats"], ids, 1)
+            ms_up = ctx.timed_upload_host_half(halves, 1)
+    print(f"half host pipeline: {ms:.2f} ms, uploads alone {ms_up:.2f} ms")
+    assert ms > 0.0 and ms_up > 0.0
+    with lolahip.Context(L, W, H, n, n_slots=n, precision=prec, kernel="fused",
+                         views_per_pass=vpp, run_length=16) as ctx:
+        ctx.set_ranges(h["ranges"])
+        for v in range(n):
+            ctx.load_view(v, v + 1, h["mats"][v], deq[v])
+        v0 = 0
+        for b in plan:
+            ctx.backproject(v0, b)
+            v0 += b
+        ref = ctx.get_volume()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    lines = _lines(got, h)
+    span = float(h["ref"].max() - h["ref"].min())
+    d = np.abs(lines - h["ref"])
+    rmse = float(np.sqrt((d ** 2).mean())) / span
+    print(f"vs FP64: max {float(d.max()) / span:.2e} rmse {rmse:.2e} of range")
+    assert rmse <= 1e-3
+    assert rmse <= HALF_RMSE[prec]

@@ -232,3 +232,43 @@ def test_writers_reject_bad_arguments_before_any_gpu_call():
     assert lib.lhWriteResultFile(None, 128, 1, 0, None, None, 1.0, 0.0, None, None, None) \
         == LH_E_ARG
     assert not os.path.exists("x.pgm")
+
+
+def _storage_exponent_ref(mx):
+    """The FP16 storage exponent restated in float32 arithmetic."""
+    mx = np.float32(mx)
+    if not mx > 0:
+        return 0
+    m, s = 0, np.float32(1.0)
+    while mx / s > np.float32(16384.0) and m < 60:
+        s, m = s * np.float32(4.0), m + 1
+    while mx / s <= np.float32(4096.0) and m > -60:
+        s, m = s * np.float32(0.25), m - 1
+    return m
+
+
+def test_host_half_views_round_like_numpy():
+    """lhHostHalfViews against numpy's own conversions: FP16 is numpy's
+    float16 (round to nearest even, subnormals, overflow) of p * 4^-m with m
+    from the view's max |p| in float32; BF16 is the top half of the float
+    rounded to nearest even.  Views with ties, subnormal-range values, signed
+    zeros, a zero view and large magnitudes."""
+    rng = np.random.default_rng(7)
+    H, W = 24, 40
+    imgs = (rng.standard_normal((5, H, W)) * np.array([1e-3, 1.0, 3e4, 1e-7, 0.0])
+            [:, None, None]).astype(np.float32)
+    imgs[1, 0, :8] = np.array([0.5, -0.0, 1 + 2 ** -11, 1 + 3 * 2 ** -11, 2 ** -20, -2 ** -24,
+                               65504.0, 1e-30], np.float32)
+    halves, exps = lolahip.host_half_views("f16", imgs)
+    for v in range(len(imgs)):
+        m = _storage_exponent_ref(np.abs(imgs[v]).max())
+        assert exps[v] == m
+        want = (imgs[v] * np.float32(4.0) ** np.float32(-m)).astype(np.float32)
+        assert np.array_equal(halves[v], want.astype(np.float16).view(np.uint16))
+    b16, e16 = lolahip.host_half_views("bf16", imgs)
+    assert not e16.any()
+    u = imgs.view(np.uint32).astype(np.uint64)
+    want = ((u + 0x7fff + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    assert np.array_equal(b16, want)
+    with pytest.raises(lolahip.LolaHipError):
+        lolahip.host_half_views("f32", imgs)
