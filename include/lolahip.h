@@ -412,6 +412,11 @@ int lhUnetSetParams(LhUnet *net, const float *host);
 int lhUnetGetGrads(LhUnet *net, float *host);
 /* dOut = net(dIn) for n cubes of D^3 floats (D a multiple of 64). */
 int lhUnetForward(LhUnet *net, const float *dIn, float *dOut, int n, int D);
+/* firstBlock != 0 (the default): lhUnetForward evaluates the first two
+ * convolutions in one kernel (the first layer's activations recomputed on
+ * the second's halo, never stored); 0: one kernel per layer, as training
+ * runs them.  The outputs are the same bit for bit. */
+int lhUnetSetFusion(LhUnet *net, int firstBlock);
 /* MSE loss of net(dIn) against dTarget, its gradients, and with lr > 0 one
  * Adam step. */
 int lhUnetTrainStep(LhUnet *net, const float *dIn, const float *dTarget, int n, int D, float lr,

@@ -783,6 +783,124 @@ __global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict
   }
 }
 
+// The network's first two convolutions at inference (enc1a: 1 -> 16
+// channels, enc1b: 16 -> 16 with the 2x2x2 pool fused) in one kernel per
+// 16 x 4 x 8 tile.  enc1a is evaluated on enc1b's whole halo (18 x 6 x 10
+// voxels, 2.1x the tile) from the input's halo of that halo (20 x 8 x 12
+// voxels, staged in LDS as bfloat16) and written into the halo buffer enc1b
+// reads (zeros outside the volume: enc1b's padding), so enc1a's 32 bytes
+// per voxel never go to HBM and come back 2.1 times as enc1b's halo.  Each
+// MFMA has the operands of conv_first_kernel (one halo voxel per A row, its
+// 27 taps then 5 zeros) and of the 16-channel path, and the roundings are
+// theirs: the outputs are the two-kernel form's bit for bit.
+__global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restrict__ in,
+    const uint16_t *__restrict__ w0, const float *__restrict__ b0, ConvLdsArgs a)
+{
+  constexpr int MT = 8;
+  constexpr int IX = kHX + 2, IY = kHY + 2, IZ = MT + 4;  // input halo of the halo
+  constexpr int IV = IX * IY * IZ;
+  constexpr int HV = kHX * kHY * (MT + 2);                // enc1b's halo voxels
+  constexpr int GROUPS = (HV + 15) / 16;                  // MFMAs of enc1a
+  __shared__ uint16_t inh[IV];
+  // input-halo index of tap (0, 0, 0) of each halo voxel (the last group's
+  // pad entries repeat the last voxel)
+  __shared__ uint16_t vbase[GROUPS * 16];
+  // enc1b's halo [MT+2][kHY][kHX][16], then the epilogue's transpose buffer
+  __shared__ __align__(16) uint16_t halo[HV * 16];
+  const int D = a.D;
+  const int txN = D / 16, tyN = D / 4, tzN = D / MT;
+  int tile = blockIdx.x;
+  if (a.tilesPerXcd > 0)
+    tile = (blockIdx.x & 7) * a.tilesPerXcd + (blockIdx.x >> 3);
+  const int tx = tile % txN;
+  int rest = tile / txN;
+  const int ty = rest % tyN;
+  rest /= tyN;
+  const int tz = rest % tzN, n = rest / tzN;
+  const int x0 = tx * 16, y0 = ty * 4, z0 = tz * MT;
+  // the input around the halo, all of a thread's loads issued before the
+  // first is used; zeros outside the volume
+  constexpr int ITER = (IV + 255) / 256;
+  float iv[ITER];
+#pragma unroll
+  for (int k = 0; k < ITER; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    iv[k] = 0.0f;
+    if (i < IV) {
+      const int gx = x0 - 2 + i % IX, gy = y0 - 2 + (i / IX) % IY, gz = z0 - 2 + i / (IX * IY);
+      if ((unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D && (unsigned)gz < (unsigned)D)
+        iv[k] = in[(((size_t)n * D + gz) * D + gy) * D + gx];
+    }
+  }
+  for (int i = threadIdx.x; i < GROUPS * 16; i += 256) {
+    const int v = min(i, HV - 1);
+    vbase[i] = (uint16_t)(((v / (kHX * kHY)) * IY + (v / kHX) % kHY) * IX + v % kHX);
+  }
+#pragma unroll
+  for (int k = 0; k < ITER; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    if (i < IV)
+      inh[i] = f2bf(iv[k]);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  {
+    // enc1a: halo voxels 16 g .. 16 g + 15 per MFMA (lane m's A row is voxel
+    // 16 g + m, its taps 8 kg .. 8 kg + 7); wave w takes groups w, w + 4, ...
+    const bf16x8 B =
+        __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(w0 + m * 32 + 8 * kg));
+    int toff[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int tap = 8 * kg + j;
+      const int tp  = tap < 27 ? tap : 0;
+      toff[j] = ((tp / 9) * IY + (tp / 3) % 3) * IX + tp % 3;
+    }
+    const float bias = b0[m];
+    for (int g = w; g < GROUPS; g += 4) {
+      const int base = vbase[16 * g + m];
+      uint16_t av[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        av[j] = (8 * kg + j < 27) ? inh[base + toff[j]] : (uint16_t)0;
+      const uint4 a4 = make_uint4((uint32_t)av[0] | ((uint32_t)av[1] << 16),
+          (uint32_t)av[2] | ((uint32_t)av[3] << 16), (uint32_t)av[4] | ((uint32_t)av[5] << 16),
+          (uint32_t)av[6] | ((uint32_t)av[7] << 16));
+      const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a4),
+          B, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+      uint16_t *hp = halo + (16 * g + 4 * kg) * 16 + m;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (16 * g + 4 * kg + i < HV)
+          hp[16 * i] = f2bf(fmaxf(acc[i] + bias, 0.0f));
+    }
+  }
+  // halo voxels outside the volume hold enc1b's zero padding, not enc1a's
+  // output there (tiles at the volume's faces only)
+  if (x0 == 0 || y0 == 0 || z0 == 0 || x0 + 16 == D || y0 + 4 == D || z0 + MT == D) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < HV * 2; i += 256) {
+      const int v  = i >> 1;
+      const int gx = x0 - 1 + v % kHX, gy = y0 - 1 + (v / kHX) % kHY,
+                gz = z0 - 1 + v / (kHX * kHY);
+      if (!((unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+              (unsigned)gz < (unsigned)D))
+        *reinterpret_cast<uint4 *>(halo + v * 16 + 8 * (i & 1)) = make_uint4(0, 0, 0, 0);
+    }
+  }
+  __syncthreads();
+  // enc1b: the 16-channel path on the computed halo, then bias, ReLU, the
+  // stores and the fused pool
+  const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
+  f32x4 acc[MT][1];
+#pragma unroll
+  for (int r = 0; r < MT; ++r)
+    acc[r][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  tail16_mfma<1, MT>(acc, halo, wrow, (size_t)16 * a.kp, 0, w, m, kg);
+  conv_epilogue<1, MT>(a, acc, halo, n, x0, y0, z0, w, lane, m, kg);
+}
+
 // Weight gradient dW[co][tap][ci] += sum_v g[v][co] * x[v + off(tap)][ci].
 // Grid: (blocks over x-rows) x 9 (dz, dy).  Per 32-voxel x segment, LDS
 // holds g transposed (gT[co][v]) and the input row at the three x shifts,
@@ -1118,6 +1236,7 @@ struct LhUnet {
   uint16_t *w0 = nullptr;  // first layer, bfloat16 [16][32]
   float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
   int step = 0;
+  int fuseFirst = 1;       // inference: enc1a + enc1b in conv_first16_kernel
   hipStream_t stream = nullptr;
   // workspace for one (n, D)
   int wsN = 0, wsD = 0;
@@ -1452,11 +1571,6 @@ int forward(LhUnet *u, const float *in, int n, int D, bool training, float *outp
   auto A = [](uint16_t *p, int ld, int ofs = 0) { Act a; a.p = p; a.ld = ld; a.ofs = ofs; return a; };
   if (training)
     pack_input_kernel<<<grid_for((size_t)n1 * 16), 256, 0, s>>>(in, u->x16, (size_t)n1);
-  {
-    const size_t tiles = (size_t)n * (D1 / 16) * (D1 / 4) * (D1 / 8);
-    conv_first_kernel<<<(unsigned)tiles, 256, 0, s>>>(in, u->w0, u->c[0].b, u->a1, D1,
-        tiles % 8 == 0 ? (int)(tiles / 8) : 0);
-  }
 #define FW(L, i, o, Dd) if ((rc = conv_fwd(s, u->c[L], i, o, Dd, n, true, true, false, false)) != LH_OK) return rc
   // the encoders' convolutions also write the pooled tensors
   PoolFuse pf1, pf2;
@@ -1464,9 +1578,32 @@ int forward(LhUnet *u, const float *in, int n, int D, bool training, float *outp
   pf1.ld = 16;
   pf2.p = u->p2;
   pf2.ld = 32;
-  if ((rc = conv_fwd(s, u->c[1], plain_in(A(u->a1, 16)), A(u->e1, 16), D1, n, true, true, false,
-           false, HeadFuse(), pf1)) != LH_OK)
-    return rc;
+  const size_t tiles1 = (size_t)n * (D1 / 16) * (D1 / 4) * (D1 / 8);
+  const int perXcd1   = tiles1 % 8 == 0 ? (int)(tiles1 / 8) : 0;
+  if (!training && u->fuseFirst) {
+    // enc1a + enc1b (+ pool) in one kernel; enc1a's activations are not kept
+    ConvLdsArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.w = u->c[1].wf;
+    a.kp = u->c[1].kpf;
+    a.bias = u->c[1].b;
+    a.cin = 16;
+    a.c0 = 16;
+    a.y = u->e1;
+    a.ldy = 16;
+    a.D = D1;
+    a.n = n;
+    a.relu = 1;
+    a.pool = pf1.p;
+    a.ldp = pf1.ld;
+    a.tilesPerXcd = perXcd1;
+    conv_first16_kernel<<<(unsigned)tiles1, 256, 0, s>>>(in, u->w0, u->c[0].b, a);
+  } else {
+    conv_first_kernel<<<(unsigned)tiles1, 256, 0, s>>>(in, u->w0, u->c[0].b, u->a1, D1, perXcd1);
+    if ((rc = conv_fwd(s, u->c[1], plain_in(A(u->a1, 16)), A(u->e1, 16), D1, n, true, true,
+             false, false, HeadFuse(), pf1)) != LH_OK)
+      return rc;
+  }
   FW(2, A(u->p1, 16), A(u->a2, 32), D2);
   if ((rc = conv_fwd(s, u->c[3], plain_in(A(u->a2, 32)), A(u->e2, 32), D2, n, true, true, false,
            false, HeadFuse(), pf2)) != LH_OK)
@@ -1765,6 +1902,14 @@ int lhUnetGetGrads(LhUnet *u, float *host)
 }
 
 // Inference: dOut (device, n*D^3 floats) = net(dIn); D a multiple of 64.
+int lhUnetSetFusion(LhUnet *u, int firstBlock)
+{
+  if (!u)
+    return lh_fail(LH_E_ARG, "lhUnetSetFusion: null network");
+  u->fuseFirst = firstBlock != 0;
+  return LH_OK;
+}
+
 int lhUnetForward(LhUnet *u, const float *dIn, float *dOut, int n, int D)
 {
   if (!u || !dIn || !dOut || n < 1)

@@ -318,10 +318,13 @@ def main(argv=None) -> int:
     ap.add_argument("what", choices=["unet", "pipeline"])
     ap.add_argument("--L", type=int, default=256)
     ap.add_argument("--reps", type=int, default=6)
+    ap.add_argument("--split-first", action="store_true",
+                    help="U-Net: the first two convolutions as two kernels")
     args = ap.parse_args(argv)
     if args.what == "unet":
         net = unet.UNet(seed=0)
         load_weights(net, DEFAULT_WEIGHTS)
+        net.set_fusion(not args.split_first)
         v = np.random.default_rng(0).normal(size=(1,) + (args.L,) * 3).astype(np.float32)
         with unet.DeviceBuffer.from_array(v) as din, unet.DeviceBuffer(v.nbytes) as dout:
             for i in range(args.reps):

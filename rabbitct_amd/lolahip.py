@@ -79,7 +79,7 @@ EXPORTS = (
     # sparse-view + learned correction
     "lhLineIntegrals", "lhForwardProject", "lhRelativeResidual", "lhExtractPatches",
     "lhUnetCreate", "lhUnetDestroy", "lhUnetParamCount", "lhUnetGetParams", "lhUnetSetParams",
-    "lhUnetGetGrads", "lhUnetForward", "lhUnetTrainStep", "lhConv3dTest", "lhConv3dWgradTest",
+    "lhUnetGetGrads", "lhUnetForward", "lhUnetSetFusion", "lhUnetTrainStep", "lhConv3dTest", "lhConv3dWgradTest",
     "lhConv3dSplitTest",
     "lhDeviceAlloc", "lhDeviceFree", "lhCopyToDevice", "lhCopyToHost",
     # before and after the path
@@ -166,6 +166,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.lhUnetSetParams.argtypes = [P, P]
     lib.lhUnetGetGrads.argtypes = [P, P]
     lib.lhUnetForward.argtypes = [P, P, P, i32, i32]
+    lib.lhUnetSetFusion.argtypes = [P, i32]
     lib.lhUnetTrainStep.argtypes = [P, P, P, i32, i32, f32, ctypes.POINTER(f32)]
     lib.lhConv3dTest.argtypes = [P, i32, i32, P, P, P, i32, i32, i32, i32]
     lib.lhConv3dWgradTest.argtypes = [P, i32, P, i32, P, i32, i32]

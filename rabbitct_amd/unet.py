@@ -150,6 +150,11 @@ class UNet:
         _check(self.lib.lhUnetGetGrads(self._h, _p(out)), "lhUnetGetGrads")
         return out
 
+    def set_fusion(self, first_block: bool) -> None:
+        """lhUnetSetFusion: the first two convolutions in one kernel at
+        inference (default) or one kernel each."""
+        _check(self.lib.lhUnetSetFusion(self._h, int(bool(first_block))), "lhUnetSetFusion")
+
     def forward(self, d_in, d_out, n: int, D: int) -> None:
         """d_out = net(d_in), device buffers of n*D^3 floats."""
         _check(self.lib.lhUnetForward(self._h, _addr(d_in), _addr(d_out), n, D), "lhUnetForward")

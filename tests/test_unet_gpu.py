@@ -259,3 +259,29 @@ def test_forward_is_translation_equivariant_d128():
     b = ys[:, 40:88, 36:92, 48:112]
     assert np.abs(a).max() > 0
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("n,D", [(1, 64), (2, 128)])
+def test_fused_first_block_equals_two_kernels(n, D):
+    """The inference form that evaluates enc1a on enc1b's halo inside one
+    kernel (lhUnetSetFusion(1), the default) gives the forward of the
+    two-kernel form bit for bit: same MFMA operands, same roundings, zeros
+    where the halo leaves the volume.  Trained weights, a smooth input plus
+    noise (so ReLU cuts both ways), two cubes at D=128 (the tiles of the
+    second cube follow the first's)."""
+    rng = np.random.default_rng(11)
+    c = (np.arange(D) - D / 2) / D
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    base = np.exp(-8 * (x ** 2 + 1.5 * y ** 2 + 2 * z ** 2))
+    vol = np.stack([base * (k + 1) + 0.2 * rng.standard_normal((D, D, D))
+                    for k in range(n)]).astype(np.float32)
+    with unet.UNet(seed=0) as net:
+        net.load(os.path.join(unet.WEIGHTS_DIR, "unet_sparse4_l256.npz"))
+        net.set_fusion(True)
+        fused = net.forward_host(vol)
+        net.set_fusion(False)
+        split = net.forward_host(vol)
+    print(f"\ncase: U-Net first block fused vs split, n={n} D={D}; "
+          f"max |out| {np.abs(fused).max():.3f}", flush=True)
+    assert np.isfinite(fused).all()
+    assert np.array_equal(fused.view(np.uint32), split.view(np.uint32))
