@@ -28,6 +28,8 @@ phantom, generated on the GPU).  Reported:
   cpu_baseline the reference's own LolaASM (and LolaOPT) from oracle/_ref,
                run by the reference runner on this host on a sample of views,
                and the same runner driving LolaHIP (-m LolaHIP -b 16)
+  l1024        the full-size workload (L=1024, 4 GiB volume, 496 views):
+               fused FP32 and FP16 at 32-voxel runs, 64 views per launch
   sparse_cnn   the second half of the metric, at L=256 (R_L=1 mm): volumes/s
                of the sparse-view + U-Net pipeline (124 of 496 views, fused
                FP32 backprojection, U-Net, held-out-view residual check with
@@ -252,6 +254,43 @@ def host_pipeline_half(args, geom, ranges, pinned, shape, prec, resident_vol):
                    "the staging launch, graph-captured fused batches; HIP events from the "
                    "first copy to the last kernel")
     return best
+
+
+def full_size_l1024(args):
+    """The full-size workload, L=1024 (R_L 0.25 mm, 4 GiB volume resident in
+    HBM), 496 views of a 30-ellipsoid phantom: fused FP32 and FP16 at the
+    shape the autotuner picks there (32-voxel runs, 64 views per launch; fixed
+    here to keep the run short), with each one's error against the other."""
+    import copy
+    a = copy.copy(args)
+    a.L, a.slab, a.run, a.tile_y, a.no_tune = 1024, None, 32, 32, True
+    geom = ScanGeometry(n_views=args.views, seed=1024).build()
+    phantom = EllipsoidPhantom(n_ell=30, seed=1024).build()
+    ranges = lolahip.compute_line_ranges(1024, geom.width, geom.height, geom.matrices, 16,
+                                         device=True)
+    out, vols = {}, {}
+    for prec in ("f32", "f16"):
+        t0 = time.time()
+        ctx, _ = make_context(a, geom, phantom, ranges, "fused", prec, 64)
+        ms, ms_launch, launches, vol = time_variant(a, ctx, args.views)
+        eff = ctx.effective_updates(args.views)
+        ctx.close()
+        vols[prec] = vol
+        out[f"fused-{prec}"] = {"ms_per_step": round(ms, 3),
+                                "gups": round(1024 ** 3 * args.views / ms / 1e6, 2),
+                                "gups_effective": round(eff / ms / 1e6, 2),
+                                "ms_per_launch": round(ms_launch, 4), "launches": launches,
+                                "shape": {"kernel": "fused", "views_per_pass": 64,
+                                          "run_length": 32, "staging": True, "tile_y": 32}}
+        log(f"L=1024 fused-{prec}: {ms:.2f} ms/step {out[f'fused-{prec}']['gups']} GUPS "
+            f"({time.time() - t0:.1f} s with setup)")
+    rng = float(vols["f32"].max() - vols["f32"].min())
+    d = vols["f16"].astype(np.float64) - vols["f32"]
+    out["fused-f16"]["max_err_vs_fused_f32"] = float(np.abs(d).max() / rng)
+    out["fused-f16"]["rmse_vs_fused_f32"] = float(np.sqrt((d ** 2).mean()) / rng)
+    out["workload"] = ("backprojection L=1024 (R_L=0.25 mm, 4 GiB volume), 496 views of "
+                       "1248x960, 30-ellipsoid phantom (seed 1024), inputs resident in HBM")
+    return out
 
 
 def cpu_model():
@@ -616,6 +655,8 @@ def main():
                     help="phantom ellipsoids (30 for the L=1024 configuration)")
     ap.add_argument("--no-h2d", action="store_true",
                     help="skip the host-memory pipeline (H2D copies inside the timed step)")
+    ap.add_argument("--no-l1024", action="store_true",
+                    help="skip the full-size L=1024 fused measurement")
     ap.add_argument("--no-sparse-cnn", action="store_true",
                     help="skip the sparse-view + CNN volumes/s measurement at L=256")
     ap.add_argument("--rct-io", action="store_true",
@@ -736,6 +777,7 @@ def main():
         "gather_roof": gather_roof(head),
         "cpu_baseline": None,
         "sparse_cnn": None if args.no_sparse_cnn else sparse_cnn(args),
+        "l1024": None if (args.no_l1024 or args.L != 512 or args.slab) else full_size_l1024(args),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, geom, phantom, ranges)

@@ -45,7 +45,8 @@ def bench_args(a) -> list:
     return [sys.executable, os.path.join(ROOT, "bench.py"), "--variants", a.variant,
             "--headline", a.variant, "--no-tune", "--vpp", str(a.vpp), "--run", str(a.run),
             "--tile-y", str(a.tile_y), "--L", str(a.L), "--views", str(a.views),
-            "--no-cpu-baseline", "--no-sparse-cnn", "--no-h2d", "--steps", "3", "--warmup", "1",
+            "--no-cpu-baseline", "--no-sparse-cnn", "--no-h2d", "--no-l1024", "--steps", "3",
+            "--warmup", "1",
             "--calib-copy-mib", "1024"]
 
 
