@@ -1,8 +1,9 @@
 """Launch-shape autotuner for LolaHIP.
 
 Times candidate shapes (voxels per thread along x, views per register pass,
-LDS staging, and for 16-voxel runs the staged tile's y extent: 16 x 32 x 8,
-16 x 16 x 16 or 16 x 64 x 4 voxels) on the views already resident in a
+LDS staging, and the tile's y extent: for staged 16-voxel runs 16 x 32 x 8,
+16 x 16 x 16 or 16 x 64 x 4 voxels, for 32-voxel runs 32 x 32 x 8 or
+32 x 64 x 4) on the views already resident in a
 context, HIP-event timed on the context's stream, and leaves the context set
 to the fastest.  The volume is restored afterwards (tuning launches
 accumulate into it)."""
@@ -23,7 +24,8 @@ def candidates(precision: str):
     """(kernel, views per pass, run length, staging, tile y) tuples."""
     out = []
     for st, run, vpp in itertools.product((True, False), RUNS, PASSES):
-        for ty in (TILES_Y if run == 16 and st else (32,)):
+        tiles = TILES_Y if run == 16 and st else (32, 64) if run == 32 else (32,)
+        for ty in tiles:
             out.append(("fused", vpp, run, st, ty))
     return out
 
@@ -31,7 +33,7 @@ def candidates(precision: str):
 def _apply(ctx, shape):
     kern, vpp, run, st = shape[:4]
     ctx.set_launch(kern, vpp, run, st)
-    ctx.set_tile(shape[4] if len(shape) > 4 else 32)
+    ctx.set_tile(shape[4] if len(shape) > 4 else 0)
 
 
 def autotune(ctx: "lolahip.Context", count: int, precision: str = "f32", reps: int = 1,

@@ -325,6 +325,11 @@ typedef Shape<32, 160, 40, 3> ShapeY32W;
 typedef Shape<16, 80, 62, 4> ShapeY16;
 // 64 x 4 tiles: wide, short footprints (224 x 21)
 typedef Shape<64, 224, 22, 4> ShapeY64;
+// 64 x 4 tiles of 32-voxel runs at two workgroups per CU (145 VGPRs): at
+// 1 mm voxels (L = 256) nearly every footprint exceeds the windows and the
+// taps come from L2, where a wave's 64 lanes on one z row read one band of
+// detector rows (a third of the L1 misses of 32 x 2 lanes)
+typedef Shape<64, 224, 22, 2> ShapeY64R32;
 
 struct StageBox {
   int iu0, iv0;   // footprint origin (pixels)
@@ -689,6 +694,23 @@ __device__ __forceinline__ void lds_run(float (&acc)[R], const float *sBase, flo
   }
 }
 
+// Voxels [r0, r0 + N) of a run at coordinates fma(r, d, o), taps from L2,
+// in scheduling groups of kFenceL2 voxels: l2_update per voxel, the same bits.
+template <int PREC, int MODE, int N, int r0, int R>
+__device__ __forceinline__ void l2_run(float (&acc)[R], __amdgpu_buffer_rsrc_t rs, uint32_t base,
+    int pitch, uint32_t row, float uo, float vo, float wo, float du, float dv, float dw, float hiu,
+    float hiv)
+{
+#pragma unroll
+  for (int r = r0; r < r0 + N; ++r) {
+    const float fr = (float)r;
+    acc[r] += l2_update<PREC, MODE>(rs, base, pitch, row, r ? fmaf(fr, du, uo) : uo,
+        r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
+    if ((r % kFenceL2<R>) == kFenceL2<R> - 1)
+      __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int R, int PREC, int STAGE, class SH>
 __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int tz, float *sImg,
     int *sAct)
@@ -884,25 +906,37 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       } else {
         // footprint larger than the LDS window (or taps from L2 by
         // configuration): L2 gathers in groups of kFenceL2 voxels with the
-        // clamp loop's arithmetic
-        LH_LOOP_ORIGIN("l2");
+        // clamp loop's arithmetic; segment parking where the box is off
+        // the detector edge and the ranges are aligned, as on the LDS paths
         KernelArgs &K = kernel_args();
         const __amdgpu_buffer_rsrc_t rs = slot_rsrc(K, slot);
         const int pitchE   = (int)(K.rowBytes >> 2);
         const uint32_t row = K.rowBytes;
-        const float ou = edge ? (float)box.iu0 : 0.0f, ov = edge ? (float)box.iv0 : 0.0f;
-        // byte offset of the box origin pixel (edge boxes: of pixel (0, 0))
-        const uint32_t base =
-            K.baseOffset + (edge ? 0u : (uint32_t)(box.iv0 * pitchE + box.iu0) * 4u);
+        if (aligned && !edge) {
+          LH_LOOP_ORIGIN("l2 clamp");
+          const uint32_t base = K.baseOffset + (uint32_t)(box.iv0 * pitchE + box.iu0) * 4u;
+          static_for<0, R / kSeg<R>>([&](auto hc) {
+            constexpr int h = decltype(hc)::value;
+            LH_SEGMENT(h);
+            l2_run<PREC, kClamp, kSeg<R>, h * kSeg<R>>(acc, rs, base, pitchE, row, us, vs, ws,
+                dus, dvs, dws, hiu, hiv);
+          });
+        } else {
+          LH_LOOP_ORIGIN("l2 edge or partial");
+          const float ou = edge ? (float)box.iu0 : 0.0f, ov = edge ? (float)box.iv0 : 0.0f;
+          // byte offset of the box origin pixel (edge boxes: of pixel (0, 0))
+          const uint32_t base =
+              K.baseOffset + (edge ? 0u : (uint32_t)(box.iv0 * pitchE + box.iu0) * 4u);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const float fr = (float)r;
-          const bool on  = r >= rlo && r < rhi;
-          acc[r] += l2_update<PREC, kEdge>(rs, base, pitchE, row,
-              on ? fmaf(fr, du, uo) : kParkUV, on ? fmaf(fr, dv, vo) : kParkUV,
-              on ? fmaf(fr, dw, wo) : kPark, hiu, hiv, ou, ov);
-          if ((r % kFenceL2<R>) == kFenceL2<R> - 1)
-            __builtin_amdgcn_sched_barrier(0);
+          for (int r = 0; r < R; ++r) {
+            const float fr = (float)r;
+            const bool on  = r >= rlo && r < rhi;
+            acc[r] += l2_update<PREC, kEdge>(rs, base, pitchE, row,
+                on ? fmaf(fr, du, uo) : kParkUV, on ? fmaf(fr, dv, vo) : kParkUV,
+                on ? fmaf(fr, dw, wo) : kPark, hiu, hiv, ou, ov);
+            if ((r % kFenceL2<R>) == kFenceL2<R> - 1)
+              __builtin_amdgcn_sched_barrier(0);
+          }
         }
       }
 #undef LH_LOOP_ORIGIN
@@ -1281,7 +1315,7 @@ struct LhContext {
   long boxTiles = 0;                // tiles per slot in dBoxes
   std::vector<int> boxR;            // tile (run length, y extent) the slot's boxes were built
                                     // for, as R | ly << 8 (0 = none)
-  uint32_t tileY = 0;               // tile y extent for 16-voxel runs (0 = 32)
+  uint32_t tileY = 0;               // tile y extent set by lhSetTile (0: by run and voxel size)
 };
 
 // Grid of one launch: one workgroup per tile, or for the persistent form as
@@ -1803,11 +1837,16 @@ static TileDims tile_dims(const LhContext *c)
     return {16, 16, kTileY16};
   if (g.runLength == 16 && c->tileY == 64)
     return {64, 4, kTileY64};
+  // 32-voxel runs spanning more than 16 mm along x (L <= 256): nearly every
+  // footprint exceeds the windows, and the L2 gathers run fastest on 64 x 4
+  // tiles at two workgroups per CU (L=256, 124 views, FP32: 4.64 -> 4.35 ms)
+  if (g.runLength == 32 && (c->tileY == 64 || (c->tileY == 0 && 32.0f * g.R_L > 16.0f)))
+    return {64, 4, kTileY64};
   // A staged 32-voxel run spanning 16 mm along x (L = 512) has footprints
   // wider than 128 pixels at some views: the 160-pixel window, 3 workgroups
   // per CU.  At 8 mm (L = 1024) the 128-pixel window fits them at 4; at 32 mm
-  // (L = 256) most exceed both windows and take L2 gathers, which run faster
-  // at 4 workgroups per CU, as does the unstaged form at every size.
+  // (L = 256) most exceed both windows (above).  The unstaged form runs at 4
+  // workgroups per CU at every size.
   if (g.runLength == 32 && g.staging && 32.0f * g.R_L > 8.0f && 32.0f * g.R_L <= 16.0f)
     return {32, 8, kTileY32W};
   return {32, 8, kTileY32};
@@ -1950,6 +1989,7 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   case 16 * 16 + kTileY32: LH_BY_PREC(16, ShapeY32); break;
   case 16 * 16 + kTileY16: LH_BY_PREC(16, ShapeY16); break;
   case 16 * 16 + kTileY64: LH_BY_PREC(16, ShapeY64); break;
+  case 32 * 16 + kTileY64: LH_BY_PREC(32, ShapeY64R32); break;
   case 32 * 16 + kTileY32: LH_BY_PREC(32, ShapeY32); break;
   case 32 * 16 + kTileY32W: LH_BY_PREC(32, ShapeY32W); break;
   default: return fail(LH_E_ARG, "no %d x %d x %d staged tile", R, td.ly, td.lz);
@@ -2519,10 +2559,10 @@ int lhSetLaunch(LhContext *c, int kernel, uint32_t viewsPerPass, uint32_t runLen
 int lhSetTile(LhContext *c, uint32_t tileY)
 {
   if (!c || (tileY != 0 && tileY != 16 && tileY != 32 && tileY != 64))
-    return fail(LH_E_ARG, "lhSetTile: tile y extent must be 16, 32 or 64 (0: 32)");
+    return fail(LH_E_ARG, "lhSetTile: tile y extent must be 16, 32 or 64 (0: automatic)");
   if (tileY == 16 && (c->slabZ0 % 16 || c->slabZ1 % 16))
     return fail(LH_E_ARG, "lhSetTile: 16 x 16 tiles need a slab of multiples of 16");
-  c->tileY = tileY == 32 ? 0 : tileY;
+  c->tileY = tileY;
   return LH_OK;
 }
 

@@ -354,7 +354,7 @@ class Context:
         cfg.staging = 1 if staging else 0
         cfg.device = device
         self.cfg = cfg
-        self.tile_y = 32
+        self.tile_y = 0
         self._h = ctypes.c_void_p()
         _check(lib.lhCreate(ctypes.byref(cfg), ctypes.byref(self._h)), "lhCreate")
         self.lib = lib
@@ -393,8 +393,10 @@ class Context:
         _check(self.lib.lhSetSlab(self._h, z0, z1), "lhSetSlab")
 
     def set_tile(self, tile_y: int) -> None:
-        """Tile y extent of staged 16-voxel runs: 32 (16 x 32 x 8 voxels, the
-        default), 16 (16 x 16 x 16) or 64 (16 x 64 x 4)."""
+        """Tile y extent: for 16-voxel runs 32 (16 x 32 x 8 voxels), 16
+        (16 x 16 x 16) or 64 (16 x 64 x 4); for 32-voxel runs 32 (32 x 32 x 8)
+        or 64 (32 x 64 x 4); 0 (the default) picks by run length and voxel
+        size (32 x 64 x 4 for 32-voxel runs at L <= 256, else y extent 32)."""
         _check(self.lib.lhSetTile(self._h, tile_y), "lhSetTile")
         self.tile_y = tile_y
 
@@ -569,11 +571,12 @@ def gather_roof(lds: bool, blocks_per_cu: int = 4, passes: int = 64) -> float:
 
 def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel="fused",
                       views_per_pass=32, run_length=16, volume=None, staging=True, slab=None,
-                      tile_y=32):
+                      tile_y=0):
     """Backproject views (N,12)/(N,H,W) on the GPU; returns the (L,L,L) float32
     volume (+= into ``volume`` when given).  ``ranges`` None = unclipped;
     ``slab`` (z0, z1) restricts the work to that z-slab; ``tile_y`` is the
-    staged tile's y extent for 16-voxel runs (16, 32 or 64)."""
+    tile's y extent (16, 32 or 64; 0: by run length and voxel size, as
+    Context.set_tile)."""
     images = np.asarray(images, dtype=np.float32)
     N, H, W = images.shape
     with Context(L, W, H, N, n_slots=N, precision=precision, kernel=kernel,
@@ -581,7 +584,7 @@ def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel=
                  clip=ranges is not None, staging=staging) as ctx:
         if slab is not None:
             ctx.set_slab(*slab)
-        if tile_y != 32:
+        if tile_y:
             ctx.set_tile(tile_y)
         if ranges is not None:
             ctx.set_ranges(ranges)
