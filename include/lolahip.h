@@ -314,10 +314,8 @@ int lhSetLaunch(LhContext *ctx, int kernel, uint32_t viewsPerPass, uint32_t runL
  * the default) for 16 x 32 x 8 voxels (x, y, z) staged through a 128 x
  * 39-pixel LDS window, 16 for 16 x 16 x 16 (an 80 x 62 window: narrow, tall
  * footprints), 64 for 16 x 64 x 4 (224 x 22: wide, short ones).  32-voxel
- * runs: 32 for 32 x 32 x 8, 64 for 32 x 64 x 4 at two workgroups per CU;
- * 0 picks 32 x 64 x 4 where a run spans more than 16 mm along x (L <= 256:
- * footprints exceed the windows and taps come from L2), else 32 x 32 x 8.
- * 8-voxel runs keep 8 x 32 x 8.  A footprint larger than its window is
+ * runs: 32 (or 0) for 32 x 32 x 8, 64 for 32 x 64 x 4.  8-voxel runs keep
+ * 8 x 32 x 8.  A footprint larger than its window is
  * gathered from L2, so every shape gives the volume LolaOPT's arithmetic
  * defines; results may differ between shapes in the last bits (each tile's
  * local coordinate origin is its footprint's).  No counterpart in the

@@ -30,6 +30,12 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-I" + INCLU
              "-munsafe-fp-atomics", "-Wall"]
 HIP_SOURCES = ["lh_device.hip", "lh_phantom.hip", "lh_unet.hip", "lh_learned.hip",
                "lh_metric.hip", "lh_roof.hip"]
+# Per-source flags.  The backprojection kernels branch only on wave-uniform
+# values inside their view loop; left unstructurized, those branches keep the
+# accumulators in place across the tap paths (structurized, each path's join
+# carried undefined values that forced a register copy of every accumulator
+# per view).
+HIP_EXTRA = {"lh_device.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"]}
 C_SOURCES = ["lh_module.c", "lh_hosthalf.c"]
 C_FLAGS = ["-O3", "-fPIC", "-std=c11", "-fopenmp", "-Wall", "-Wextra", "-I" + INCLUDE]
 
@@ -46,7 +52,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in HIP_SOURCES:
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        _run([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in C_SOURCES:
         o = os.path.join(BUILD, src.replace(".c", ".o"))

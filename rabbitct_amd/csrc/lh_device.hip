@@ -323,13 +323,8 @@ typedef Shape<32, 160, 40, 3> ShapeY32W;
 // 16 x 16 tiles: narrow, tall footprints (80 x 60: at L=512 every 16-voxel
 // run's fits)
 typedef Shape<16, 80, 62, 4> ShapeY16;
-// 64 x 4 tiles: wide, short footprints (224 x 21)
+// 64 x 4 tiles: wide, short footprints (224 x 21); for 16- and 32-voxel runs
 typedef Shape<64, 224, 22, 4> ShapeY64;
-// 64 x 4 tiles of 32-voxel runs at two workgroups per CU (145 VGPRs): at
-// 1 mm voxels (L = 256) nearly every footprint exceeds the windows and the
-// taps come from L2, where a wave's 64 lanes on one z row read one band of
-// detector rows (a third of the L1 misses of 32 x 2 lanes)
-typedef Shape<64, 224, 22, 2> ShapeY64R32;
 
 struct StageBox {
   int iu0, iv0;   // footprint origin (pixels)
@@ -439,6 +434,10 @@ __global__ void box_kernel(const ViewParams *__restrict__ views, int firstSlot, 
 // (s_waitcnt vmcnt(0) and a barrier).
 // Lanes whose 16 bytes lie at or beyond the footprint width fw in their row
 // are masked off, so the copies fetch about fw, not PITCH, pixels per row.
+// a source offset beyond every slot (slots are < 2 GiB): the buffer bounds
+// check returns zeros without a memory access
+constexpr uint32_t kNoSource = 0x80000000u;
+
 template <class SH>
 struct ChunkOffsets {
   static_assert(!SH::POW2 || 1024 % SH::PITCH == 0, "whole rows per 4 chunks");
@@ -495,18 +494,27 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
     // ... in a rolled loop, and the lower half-wave alone a last odd row (the
     // window's last row too when FH is odd).  Measured 2.5% faster per step
     // than a loop over chunk counts at L=512.
+    // The lane masks are source offsets past the slot (the buffer bounds
+    // return zeros, fetched from nowhere) rather than branches: the view
+    // loop then has only wave-uniform branches.  A last odd row is copied
+    // by all 64 lanes in two 4-byte loads (a 16-byte load's upper half-wave
+    // would land in the row past the window).
     static_assert(SH::PITCH == 128, "row pairs");
-    if (co.col(0) < b.fw) {
-      uint32_t off = base + co.lane0;
-      int r0       = 2 * wv;
-      for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += co.step)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-            (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0,
-            0, 0);
-      if (r0 < b.fh && lane < 32)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-            (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0,
-            0, 0);
+    uint32_t off = co.col(0) < b.fw ? base + co.lane0 : kNoSource;
+    int r0       = 2 * wv;
+    for (; r0 + 1 < b.fh; r0 += 2 * kWaves, off += co.step)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 16, (int)off, 0, 0,
+          0);
+    if (r0 < b.fh) {
+      const uint32_t row = base + (uint32_t)(r0 * pitch) * 4u + 4u * (uint32_t)lane;
+      const uint32_t lo  = lane < b.fw ? row : kNoSource;
+      const uint32_t hi  = 64 + lane < b.fw ? row + 256u : kNoSource;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH), 4, (int)lo, 0, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + r0 * SH::PITCH + 64), 4, (int)hi, 0,
+          0, 0);
     }
   } else {
     // whole chunks only (FH keeps need <= CHUNKS); the opaque copy of the
@@ -520,10 +528,9 @@ __device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const Sta
       const int ch = w + kWaves * k;
       if (ch >= need)
         break;
-      if (co.col(k) < b.fw)
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
-            (__attribute__((address_space(3))) void *)(buf + 256 * ch), 16,
-            (int)(base + co.at(k)), 0, 0, 0);
+      const uint32_t src = co.col(k) < b.fw ? base + co.at(k) : kNoSource;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(buf + 256 * ch), 16, (int)src, 0, 0, 0);
     }
   }
 }
@@ -783,8 +790,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     int lo_n = 0, hi_n = 0;
     decode(rr_nxt, lo_n, hi_n);
     const bool actNext = hasNext && __any(lo_n < hi_n);
-    if (lane == 0)
-      sAct[((v + 1) & 1) * kWaves + wv] = actNext ? 1 : 0;
+    sAct[((v + 1) & 1) * kWaves + wv] = actNext ? 1 : 0; // the same value from every lane
     __syncthreads();
     const int *an = sAct + ((v + 1) & 1) * kWaves;
     const bool blockActNext = __builtin_amdgcn_readfirstlane(an[0] | an[1] | an[2] | an[3]) != 0;
@@ -806,7 +812,7 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
     lo_c              = lo_n;
     hi_c              = hi_n;
     const bool active = rlo < rhi;
-    if (blockAct && __any(active)) {
+    if (blockAct && __builtin_amdgcn_readfirstlane(__any(active)) != 0) {
       touched |= active;
       // FP64 per-line origin A (x0, y, z, 1) relative to the box origin:
       // u' = u - iu0 w, v' = v - iv0 w in FP64, then rounded once to FP32; the
@@ -837,7 +843,8 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
       // all out: such waves park whole segments (six selects per segment) on
       // the interior and clamp paths; other waves take the edge path, which
       // parks voxel by voxel.
-      const bool aligned = __all(!active || ((rlo % kSeg<R>) == 0 && (rhi % kSeg<R>) == 0));
+      const bool aligned = __builtin_amdgcn_readfirstlane(
+          __all(!active || ((rlo % kSeg<R>) == 0 && (rhi % kSeg<R>) == 0))) != 0;
       // Each update loop starts from its own opaque copy of the run origin
       // (an empty asm naming the loop): the loops then share no leading
       // instructions, which the compiler would otherwise hoist (with their
@@ -1315,7 +1322,7 @@ struct LhContext {
   long boxTiles = 0;                // tiles per slot in dBoxes
   std::vector<int> boxR;            // tile (run length, y extent) the slot's boxes were built
                                     // for, as R | ly << 8 (0 = none)
-  uint32_t tileY = 0;               // tile y extent set by lhSetTile (0: by run and voxel size)
+  uint32_t tileY = 0;               // tile y extent set by lhSetTile (0: the run length's default)
 };
 
 // Grid of one launch: one workgroup per tile, or for the persistent form as
@@ -1837,16 +1844,13 @@ static TileDims tile_dims(const LhContext *c)
     return {16, 16, kTileY16};
   if (g.runLength == 16 && c->tileY == 64)
     return {64, 4, kTileY64};
-  // 32-voxel runs spanning more than 16 mm along x (L <= 256): nearly every
-  // footprint exceeds the windows, and the L2 gathers run fastest on 64 x 4
-  // tiles at two workgroups per CU (L=256, 124 views, FP32: 4.64 -> 4.35 ms)
-  if (g.runLength == 32 && (c->tileY == 64 || (c->tileY == 0 && 32.0f * g.R_L > 16.0f)))
+  if (g.runLength == 32 && c->tileY == 64)
     return {64, 4, kTileY64};
   // A staged 32-voxel run spanning 16 mm along x (L = 512) has footprints
   // wider than 128 pixels at some views: the 160-pixel window, 3 workgroups
   // per CU.  At 8 mm (L = 1024) the 128-pixel window fits them at 4; at 32 mm
-  // (L = 256) most exceed both windows (above).  The unstaged form runs at 4
-  // workgroups per CU at every size.
+  // (L = 256) most exceed both windows and take L2 gathers, which run fastest
+  // at 4 workgroups per CU, as does the unstaged form at every size.
   if (g.runLength == 32 && g.staging && 32.0f * g.R_L > 8.0f && 32.0f * g.R_L <= 16.0f)
     return {32, 8, kTileY32W};
   return {32, 8, kTileY32};
@@ -1989,7 +1993,7 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   case 16 * 16 + kTileY32: LH_BY_PREC(16, ShapeY32); break;
   case 16 * 16 + kTileY16: LH_BY_PREC(16, ShapeY16); break;
   case 16 * 16 + kTileY64: LH_BY_PREC(16, ShapeY64); break;
-  case 32 * 16 + kTileY64: LH_BY_PREC(32, ShapeY64R32); break;
+  case 32 * 16 + kTileY64: LH_BY_PREC(32, ShapeY64); break;
   case 32 * 16 + kTileY32: LH_BY_PREC(32, ShapeY32); break;
   case 32 * 16 + kTileY32W: LH_BY_PREC(32, ShapeY32W); break;
   default: return fail(LH_E_ARG, "no %d x %d x %d staged tile", R, td.ly, td.lz);

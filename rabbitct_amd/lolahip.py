@@ -395,8 +395,7 @@ class Context:
     def set_tile(self, tile_y: int) -> None:
         """Tile y extent: for 16-voxel runs 32 (16 x 32 x 8 voxels), 16
         (16 x 16 x 16) or 64 (16 x 64 x 4); for 32-voxel runs 32 (32 x 32 x 8)
-        or 64 (32 x 64 x 4); 0 (the default) picks by run length and voxel
-        size (32 x 64 x 4 for 32-voxel runs at L <= 256, else y extent 32)."""
+        or 64 (32 x 64 x 4); 0 (the default): 32."""
         _check(self.lib.lhSetTile(self._h, tile_y), "lhSetTile")
         self.tile_y = tile_y
 
@@ -575,8 +574,7 @@ def backproject_views(L, matrices, images, ranges=None, precision="f32", kernel=
     """Backproject views (N,12)/(N,H,W) on the GPU; returns the (L,L,L) float32
     volume (+= into ``volume`` when given).  ``ranges`` None = unclipped;
     ``slab`` (z0, z1) restricts the work to that z-slab; ``tile_y`` is the
-    tile's y extent (16, 32 or 64; 0: by run length and voxel size, as
-    Context.set_tile)."""
+    tile's y extent (16, 32 or 64; 0: the default, as Context.set_tile)."""
     images = np.asarray(images, dtype=np.float32)
     N, H, W = images.shape
     with Context(L, W, H, N, n_slots=N, precision=precision, kernel=kernel,

@@ -3,8 +3,8 @@
 The fused kernel stages each tile's per-view detector footprint in LDS; the
 tile shape fixes the window: 16 x 32 x 8 voxels (x, y, z) with a 128 x 39
 window, 16 x 16 x 16 with 80 x 62, 16 x 64 x 4 with 224 x 22, 32-voxel runs
-at L <= 512 (32 x 32 x 8) with 160 x 39, and 32 x 64 x 4 (224 x 22, two
-workgroups per CU, the 32-voxel default at L <= 256).  For every shape:
+at L <= 512 (32 x 32 x 8) with 160 x 39, and 32 x 64 x 4 with 224 x 22.  For
+every shape:
   - the staged kernel equals the same tiles with every tap gathered from L2
     bit for bit (both fix the local coordinate origin at the tile's
     footprint box, so the arithmetic is the same);
