@@ -108,6 +108,12 @@ struct BpArgs {
   const int2 *boxes;       // staged kernel: footprint box per (tile, slot), [tile][boxSlots]
   int boxSlots;            // slots per tile row of boxes
   uint32_t boxBytes;       // bytes of the boxes array (< 4 GiB)
+  // persistent form: XCD x walks entries [xcdTile[x], xcdTile[x + 1]) of
+  // the tile list `tiles` (tile indices of the launch, z-major, relative to
+  // tileZ0), or of the tiles themselves where `tiles` is null
+  const int *tiles;
+  int xcdTile[9];
+  int *taken;              // persistent form: entries taken per XCD range (zero at launch)
 };
 
 // Per-view parameters are read through the constant address space: uniform
@@ -968,9 +974,12 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
 }
 
 // Grid form: one workgroup per tile.  Persistent form: a grid sized to the
-// chip walks the tiles; tile t goes to XCD (t mod 8) in dispatch order, so the
-// tiles are renumbered such that each XCD's share is one contiguous z-slab and
-// its L2 serves one band of detector rows.
+// chip walks the tiles that hold work (tile_list).  Workgroups run on the 8
+// XCDs in turn in dispatch order, so blockIdx mod 8 names one XCD's
+// workgroups; each XCD's range is one contiguous part of the z-major list
+// (one z-slab, so its L2 serves one band of detector rows), the ranges cut
+// to equal work, and a workgroup takes tiles from another XCD's range only
+// when its own is used up.
 // A must stay the kernel's only argument: kernel_args() reads it back from
 // offset 0 of the kernarg segment.
 // SH::OCC: workgroups per CU the register and LDS budget is sized for
@@ -984,21 +993,46 @@ __global__ __launch_bounds__(kWaves *kWave, SH::OCC) void bp_kernel(BpArgs A)
     bp_tile_lds<R, PREC, STAGE, SH>(A, blockIdx.x, blockIdx.y, A.tileZ0 + blockIdx.z, sImg, sAct);
     return;
   }
-  const int nTiles  = A.tilesX * A.tilesY * A.tilesZ;
-  const int perXcd  = (nTiles + 7) / 8;
-  const int xcd     = blockIdx.x & 7;
-  const int local   = blockIdx.x >> 3;
-  const int stride  = (gridDim.x + 7 - xcd) / 8; // workgroups of this XCD
-  for (int k = local; k < perXcd; k += stride) {
-    const int t = xcd * perXcd + k;
-    if (t >= nTiles)
+  // Each workgroup takes the next entry of its XCD's range (one atomic per
+  // tile, lane 0, handed to the other waves through LDS), then, once that
+  // range is used up, of the other XCDs' ranges: tiles of unequal work
+  // (clipped lines, the cylinder's edge) no longer leave workgroups idle
+  // while others still hold a static share.
+  __shared__ int sNext;
+  const int xcd = blockIdx.x & 7;
+  for (;;) {
+    __syncthreads(); // the previous tile's last LDS reads (footprints, votes, sNext) are done
+    if (threadIdx.x == 0) {
+      int e = -1;
+      for (int k = 0; k < 8 && e < 0; ++k) {
+        const int x = (xcd + k) & 7;
+        const int n = A.xcdTile[x + 1] - A.xcdTile[x];
+        if (n > 0) {
+          const int i = atomicAdd(A.taken + x, 1);
+          if (i < n)
+            e = A.xcdTile[x] + i;
+        }
+      }
+      sNext = e;
+    }
+    __syncthreads();
+    const int e = __builtin_amdgcn_readfirstlane(sNext);
+    if (e < 0)
       break;
+    const int t  = A.tiles ? A.tiles[e] : e;
     const int tx = t % A.tilesX;
     const int ty = (t / A.tilesX) % A.tilesY;
     const int tz = A.tileZ0 + t / (A.tilesX * A.tilesY);
-    __syncthreads(); // the previous tile's last LDS reads (footprints, votes) are done
     bp_tile_lds<R, PREC, STAGE, SH>(A, tx, ty, tz, sImg, sAct);
   }
+}
+
+// zeroes the persistent walk's per-XCD counters before a launch (a kernel,
+// not a memset, so that captured batches replay it as a kernel node)
+__global__ void zero_taken_kernel(int *taken)
+{
+  if (threadIdx.x < 8)
+    taken[threadIdx.x] = 0;
 }
 
 // ---- projection staging -------------------------------------------------
@@ -1281,6 +1315,13 @@ int scratch_boxes(int dev, int2 **out)
 
 // ---- context ---------------------------------------------------------------
 
+// The tiles of one launch shape that hold a voxel inside some view's line
+// ranges (persistent walk order, on the device) and the per-XCD cuts.
+struct TileList {
+  int *dTiles = nullptr;
+  int cut[9]  = {};
+};
+
 struct LhContext {
   LhConfig cfg{};
   hipStream_t stream = nullptr;
@@ -1289,6 +1330,7 @@ struct LhContext {
   ViewParams *dViews = nullptr;
   uint8_t *dSlots    = nullptr;
   uint32_t *dMax     = nullptr;     // one per slot
+  int *dTaken        = nullptr;     // the persistent walk's per-XCD counters (8)
   // host-upload ring: pinned host buffers (allocated on first use) and device
   // staging buffers of W*H floats; the H2D copies run on upStream
   hipStream_t upStream = nullptr;
@@ -1315,6 +1357,13 @@ struct LhContext {
   uint32_t baseOffset  = 0;
   int numCU = 0;
   std::vector<uint64_t> effPerView; // by id-1
+  // per line (z L + y): the union over views of the line ranges, [lineLo,
+  // lineHi) (empty when lineLo >= lineHi; no vectors: ranges not counted),
+  // and the voxels inside the ranges summed over views
+  std::vector<int> lineLo, lineHi;
+  std::vector<uint32_t> lineWork;
+  // tiles holding work per (run, tile y, tile z, slab z0, slab z1)
+  std::map<std::tuple<int, int, int, uint32_t, uint32_t>, TileList> tileLists;
   std::vector<int> slotId;          // id held by each slot (0 = empty)
   uint32_t slabZ0 = 0, slabZ1 = 0;  // z-slab of the volume backprojected ([0, L) by default)
   int2 *dBoxes  = nullptr;          // staged kernel: footprint box per (slot, tile)
@@ -1337,8 +1386,8 @@ static void launch_shape(LhContext *c, const BpArgs &A)
                            bp_kernel<R, PREC, STAGE, SH>, kWaves * kWave, 0) != hipSuccess ||
                           perCU < 1))
       perCU = SH::OCC;
-    grid = dim3((unsigned)std::min<long>((long)c->numCU * perCU,
-        (long)A.tilesX * A.tilesY * A.tilesZ));
+    grid = dim3((unsigned)std::max<long>(1, std::min<long>((long)c->numCU * perCU,
+        (long)A.tilesX * A.tilesY * A.tilesZ)));
   }
   bp_kernel<R, PREC, STAGE, SH><<<grid, dim3(kWaves * kWave), 0, c->stream>>>(A);
 }
@@ -1351,6 +1400,7 @@ struct TileDims {
   int ly, lz, kind;
 };
 static TileDims tile_dims(const LhContext *c);
+static void drop_tile_lists(LhContext *c);
 
 const LhConfig &lh_config(const LhContext *c) { return c->cfg; }
 hipStream_t lh_stream(const LhContext *c) { return c->stream; }
@@ -1459,6 +1509,7 @@ static int create_buffers(LhContext *c)
   LH_HIP(hipMalloc(&c->dSlots, c->slotBytes * cfg->nSlots));
   LH_HIP(hipMemsetAsync(c->dSlots, 0, c->slotBytes * cfg->nSlots, c->stream));
   LH_HIP(hipMalloc(&c->dMax, cfg->nSlots * sizeof(uint32_t)));
+  LH_HIP(hipMalloc(&c->dTaken, 8 * sizeof(int)));
   // one ring entry per view of a register pass (2..32), so a whole batch can
   // be in flight while the previous one is backprojected
   LH_HIP(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
@@ -1506,7 +1557,9 @@ int lhDestroy(LhContext *c)
   for (auto &kv : c->graphCache)
     (void)hipGraphExecDestroy(kv.second);
   (void)hipFree(c->dMax);
+  (void)hipFree(c->dTaken);
   (void)hipFree(c->dBoxes);
+  drop_tile_lists(c);
   for (int k = 0; k < c->nRing; ++k) {
     (void)hipFree(c->dRing[k]);
     if (c->hRing[k])
@@ -1524,20 +1577,124 @@ int lhDestroy(LhContext *c)
   return LH_OK;
 }
 
+static void drop_graphs(LhContext *c);
+
+// Voxels inside the line ranges per view, and per line the union of the
+// views' ranges.  The persistent walk's tile lists follow the unions, so
+// lists and the batches captured with them are dropped.
 static void count_effective(LhContext *c, const int *packed)
 {
-  const uint64_t LL = (uint64_t)c->cfg.L * c->cfg.L;
+  const uint32_t L  = c->cfg.L;
+  const uint64_t LL = (uint64_t)L * L;
+  drop_graphs(c);
+  drop_tile_lists(c);
+  c->lineLo.assign(LL, (int)L);
+  c->lineHi.assign(LL, 0);
+  c->lineWork.assign(LL, 0);
   for (uint32_t n = 0; n < c->cfg.nViews; ++n) {
     uint64_t s = 0;
     for (uint64_t i = 0; i < LL; ++i) {
       const int v     = packed[n * LL + i];
       const int start = std::max((int)(short)(v & 0xffff), 0);
-      const int stop  = std::min(v >> 16, (int)c->cfg.L);
-      if (stop > start)
+      const int stop  = std::min(v >> 16, (int)L);
+      if (stop > start) {
         s += (uint64_t)(stop - start);
+        c->lineLo[i] = std::min(c->lineLo[i], start);
+        c->lineHi[i] = std::max(c->lineHi[i], stop);
+        c->lineWork[i] += (uint32_t)(stop - start);
+      }
     }
     c->effPerView[n] = s;
   }
+}
+
+// The persistent walk of a launch shape (BpArgs::tiles, xcdTile): the tiles
+// whose lines' range unions meet the tile's x extent, z-major, cut into 8
+// contiguous ranges of equal work (a tile's work: its lines' voxels inside
+// the ranges over all views, each line's share by the overlap of the tile
+// with the line's union, plus 1/16 of a full tile per view for the view
+// loop an idle tile still runs).  Without it, a volume whose top and bottom slabs project past
+// the detector (L=512: work in the middle ~70% of z) left two XCDs walking
+// mostly idle tiles and four at 1.41x the mean work; walking the idle tiles
+// costs a view loop each (range loads and a barrier per view).  Built on the
+// first launch of a shape after the ranges are set (mayAllocate); otherwise,
+// or without clipping, every tile in 8 even ranges.
+static void drop_tile_lists(LhContext *c)
+{
+  for (auto &kv : c->tileLists)
+    (void)hipFree(kv.second.dTiles);
+  c->tileLists.clear();
+}
+
+static int tile_list(LhContext *c, int R, const TileDims &td, int tilesX, int tilesY,
+    int tileZ0, int tilesZ, bool mayAllocate, const int **tiles, int *cut)
+{
+  *tiles = nullptr;
+  const long nAll = (long)tilesX * tilesY * tilesZ;
+  for (int x = 0; x <= 8; ++x)
+    cut[x] = (int)(nAll * x / 8);
+  if (!c->cfg.clip || c->lineLo.empty())
+    return LH_OK;
+  const auto key = std::make_tuple(R, td.ly, td.lz, c->slabZ0, c->slabZ1);
+  auto it = c->tileLists.find(key);
+  if (it == c->tileLists.end()) {
+    if (!mayAllocate)
+      return LH_OK;
+    const int L = (int)c->cfg.L;
+    std::vector<int> list;
+    std::vector<double> work;
+    const double idle = (double)R * td.ly * td.lz / 16.0 * c->cfg.nViews;
+    for (int r = 0; r < tilesZ; ++r)
+      for (int ty = 0; ty < tilesY; ++ty)
+        for (int tx = 0; tx < tilesX; ++tx) {
+          bool inside = false;
+          double w    = 0.0;
+          for (int z = (tileZ0 + r) * td.lz; z < (tileZ0 + r + 1) * td.lz; ++z)
+            for (int y = ty * td.ly; y < (ty + 1) * td.ly; ++y) {
+              const size_t line = (size_t)z * L + y;
+              const int lo = std::max(c->lineLo[line], tx * R);
+              const int hi = std::min(c->lineHi[line], tx * R + R);
+              if (hi > lo) {
+                inside = true;
+                w += (double)c->lineWork[line] * (hi - lo) /
+                     (double)(c->lineHi[line] - c->lineLo[line]);
+              }
+            }
+          if (inside) {
+            list.push_back(tx + tilesX * (ty + tilesY * r));
+            work.push_back(w + idle);
+          }
+        }
+    TileList tl;
+    double total = 0.0;
+    for (double w : work)
+      total += w;
+    const int n = (int)list.size();
+    tl.cut[0] = 0;
+    tl.cut[8] = n;
+    double acc = 0.0;
+    int i      = 0;
+    for (int x = 1; x < 8; ++x) {
+      while (i < n && acc + 0.5 * work[(size_t)i] < total * x / 8.0)
+        acc += work[(size_t)i++];
+      tl.cut[x] = i;
+    }
+    if (n > 0) {
+      LH_HIP(hipMalloc(&tl.dTiles, (size_t)n * sizeof(int)));
+      const hipError_t e =
+          hipMemcpy(tl.dTiles, list.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(tl.dTiles);
+        return fail(LH_E_HIP, "tile list: %s", hipGetErrorString(e));
+      }
+    }
+    it = c->tileLists.emplace(key, tl).first;
+  }
+  // an empty list (no work in any view) walks nothing: every cut is 0
+  *tiles = it->second.dTiles;
+  for (int x = 0; x <= 8; ++x)
+    cut[x] = it->second.cut[x];
+  return LH_OK;
 }
 
 int lhSetRanges(LhContext *c, const LhLineRange *ranges)
@@ -1957,6 +2114,17 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
   A.tileZ0      = (int)c->slabZ0 / td.lz;
   A.tilesZ      = (int)(c->slabZ1 - c->slabZ0) / td.lz;
   A.persistent  = g.kernel == LH_KERNEL_FUSED;
+  A.tiles       = nullptr;
+  for (int x = 0; x <= 8; ++x)
+    A.xcdTile[x] = 0;
+  if (A.persistent) {
+    // the allocation-free entry point (extBoxes) walks a list only where an
+    // earlier launch of this shape built it
+    const int rc = tile_list(c, R, td, A.tilesX, A.tilesY, A.tileZ0, A.tilesZ,
+        extBoxes == nullptr, &A.tiles, A.xcdTile);
+    if (rc != LH_OK)
+      return rc;
+  }
   A.boxSlots    = (int)g.nSlots;
   if (extBoxes) {
     A.views     = c->dViews + first;
@@ -1988,6 +2156,11 @@ static int launch_bp(LhContext *c, uint32_t first, uint32_t count, bool forceBox
     else launch_shape<RR, LH_PREC_F16, 0, SH>(c, A);                                      \
     break;                                                                               \
   }
+  A.taken = c->dTaken;
+  if (A.persistent) {
+    zero_taken_kernel<<<1, 64, 0, c->stream>>>(c->dTaken);
+    LH_HIP(hipGetLastError());
+  }
   switch (R * 16 + td.kind) {
   case 8 * 16 + kTileY32: LH_BY_PREC(8, ShapeY32); break;
   case 16 * 16 + kTileY32: LH_BY_PREC(16, ShapeY32); break;
@@ -2017,9 +2190,15 @@ static int graph_backproject(LhContext *c, uint32_t first, uint32_t count, uint3
   auto it = c->graphCache.find(key);
   if (it == c->graphCache.end()) {
     {
-      // allocate the box table outside the capture
-      const int R  = (int)g.runLength;
-      const int rc = box_storage(c, (long)(g.L / R) * (g.L / td.ly) * (g.L / td.lz));
+      // allocate the box table and the tile list outside the capture
+      const int R = (int)g.runLength;
+      int rc      = box_storage(c, (long)(g.L / R) * (g.L / td.ly) * (g.L / td.lz));
+      if (rc == LH_OK && g.kernel == LH_KERNEL_FUSED) {
+        const int *tiles = nullptr;
+        int cut[9];
+        rc = tile_list(c, R, td, (int)g.L / R, (int)g.L / td.ly, (int)c->slabZ0 / td.lz,
+            (int)(c->slabZ1 - c->slabZ0) / td.lz, true, &tiles, cut);
+      }
       if (rc != LH_OK)
         return rc;
     }
