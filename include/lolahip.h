@@ -207,7 +207,10 @@ int lhBackproject(LhContext *ctx, uint32_t firstSlot, uint32_t count);
  * graph replay).  Returns LH_E_ARG when one pass needs more than the table's
  * 4 Mi boxes (views per pass x tiles, tiles = (L/runLength) x (L/32) x
  * (L/8): at most 128 views per pass at L=512 with 16-voxel runs).  Calls
- * that overlap on one GPU share that table and race. */
+ * that overlap on one GPU share that table and race.  The persistent tile
+ * walk uses the list of tiles holding work that an earlier lhBackproject of
+ * the same launch shape built; without one it walks every tile (the same
+ * result, slower where the volume's ends are clipped). */
 int lhBackprojectNoAlloc(LhContext *ctx, uint32_t firstSlot, uint32_t count);
 
 /* Runs `reps` times the backprojection of all slots [0, count) in passes of
