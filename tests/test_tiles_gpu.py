@@ -137,3 +137,63 @@ def test_autotune_covers_tile_shapes(l512_phantom):
         assert len(r["ms"]) == 4
         assert r["best"]["tile_y"] in (16, 32, 64)
         assert not ctx.get_volume().any()   # the tuning launches are undone
+
+
+def _z_band(ranges, L, z0, z1):
+    """The ranges with every line outside z in [z0, z1) emptied."""
+    r = np.array(ranges, copy=True)
+    z = np.arange(L * L) // L
+    r.reshape(r.shape[0], L * L, -1)[:, (z < z0) | (z >= z1), :] = 0
+    return r
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_tile_walk_follows_range_changes(graphs):
+    """The persistent walk covers only the tiles whose lines hold voxels
+    inside some view's ranges (a list per launch shape built from the
+    ranges); new ranges drop that list and the batches captured with it.
+    Backprojecting with ranges confined to a z-band, then with the full
+    ranges, gives each time what a fresh context with only those ranges
+    gives, bit for bit."""
+    L, n = 256, 6
+    geom, imgs, ranges = _case(L, n, 79, noise=True)
+    narrow = _z_band(ranges, L, 96, 104)
+    print(f"\ncase: L={L} {n} views, tile walk after range changes, graphs={graphs}", flush=True)
+    kw = dict(kernel="fused", views_per_pass=n, run_length=32)
+    with lolahip.Context(L, geom.width, geom.height, n, n_slots=n, **kw) as ctx:
+        ctx.set_graphs(graphs)
+        ctx.set_ranges(narrow)
+        for v in range(n):
+            ctx.load_view(v, v + 1, geom.matrices[v], imgs[v])
+        ctx.backproject(0, n)
+        first = ctx.get_volume()
+        ctx.zero_volume()
+        ctx.set_ranges(ranges)
+        ctx.backproject(0, n)
+        second = ctx.get_volume()
+    ref_narrow = lolahip.backproject_views(L, geom.matrices, imgs, narrow, **kw)
+    ref_full = lolahip.backproject_views(L, geom.matrices, imgs, ranges, **kw)
+    assert not first[:96].any() and not first[104:].any() and first[96:104].any()
+    assert np.array_equal(first.view(np.uint32), ref_narrow.view(np.uint32))
+    assert np.array_equal(second.view(np.uint32), ref_full.view(np.uint32))
+
+
+def test_no_alloc_walk_before_any_tile_list():
+    """lhBackprojectNoAlloc on a context whose launch shape has no tile list
+    yet walks every tile (it may not allocate one): the same volume, bit for
+    bit, as lhBackproject, which then builds the list."""
+    L, n = 256, 5
+    geom, imgs, ranges = _case(L, n, 83, noise=True)
+    print(f"\ncase: L={L} {n} views, no-alloc walk without a tile list", flush=True)
+    with lolahip.Context(L, geom.width, geom.height, n, n_slots=n, kernel="fused",
+                         views_per_pass=n, run_length=16) as ctx:
+        ctx.set_ranges(ranges)
+        for v in range(n):
+            ctx.load_view(v, v + 1, geom.matrices[v], imgs[v])
+        ctx.backproject_no_alloc(0, n)
+        walked_all = ctx.get_volume()
+        ctx.zero_volume()
+        ctx.backproject(0, n)
+        listed = ctx.get_volume()
+    assert np.abs(listed).max() > 0
+    assert np.array_equal(walked_all.view(np.uint32), listed.view(np.uint32))
