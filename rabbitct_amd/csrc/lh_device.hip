@@ -671,14 +671,15 @@ constexpr int kAhead = R >= 32 ? 1 : 0;
 // another order: the same bits.
 template <int PREC, int MODE, class SH, int N, int r0, int R>
 __device__ __forceinline__ void lds_run(float (&acc)[R], const float *sBase, float uo, float vo,
-    float wo, float du, float dv, float dw, float hiu, float hiv)
+    float wo, float du, float dv, float dw, float hiu, float hiv, float ou = 0.0f,
+    float ov = 0.0f)
 {
   if constexpr (kAhead<R> == 0) {
 #pragma unroll
     for (int r = r0; r < r0 + N; ++r) {
       const float fr = (float)r;
       acc[r] += lds_update<PREC, MODE, SH>(sBase, r ? fmaf(fr, du, uo) : uo,
-          r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
+          r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, ou, ov);
       if ((r % kFence<PREC>) == kFence<PREC> - 1)
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -687,7 +688,7 @@ __device__ __forceinline__ void lds_run(float (&acc)[R], const float *sBase, flo
   auto fetch = [&](int r) {
     const float fr = (float)r;
     return lds_fetch<PREC, MODE, SH>(sBase, r ? fmaf(fr, du, uo) : uo,
-        r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, 0.0f, 0.0f);
+        r ? fmaf(fr, dv, vo) : vo, r ? fmaf(fr, dw, wo) : wo, hiu, hiv, ou, ov);
   };
   LdsTaps t[D];
 #pragma unroll
@@ -894,6 +895,17 @@ __device__ __forceinline__ void bp_tile_lds(const BpArgs &A, int tx, int ty, int
             LH_SEGMENT(h);
             lds_run<PREC, kClamp, SH, kSeg<R>, h * kSeg<R>>(acc, sBuf, us, vs, ws, dus, dvs, dws,
                 hiu, hiv);
+          });
+        } else if (aligned) {
+          // detector-edge boxes (global truncation) with whole segments parked
+          LH_LOOP_ORIGIN("lds edge");
+          const float ou = (float)box.iu0, ov = (float)box.iv0;
+          const float *sB = sBuf - (box.iv0 * SH::PITCH + box.iu0);
+          static_for<0, R / kSeg<R>>([&](auto hc) {
+            constexpr int h = decltype(hc)::value;
+            LH_SEGMENT(h);
+            lds_run<PREC, kEdge, SH, kSeg<R>, h * kSeg<R>>(acc, sB, us, vs, ws, dus, dvs, dws, hiu,
+                hiv, ou, ov);
           });
         } else {
           LH_LOOP_ORIGIN("lds edge or partial");
