@@ -509,7 +509,7 @@ def sparse_cnn(args):
     # compete with the staged form)
     fused_ms = {}
     for shape in (("fused", 64, 32, True), ("fused", 32, 32, True), ("fused", 64, 16, True),
-                  ("fused", 64, 32, False)):
+                  ("fused", 32, 16, True), ("fused", 64, 32, False)):
         fused_ms[shape], _ = timed(lambda: scan.full_view(*shape), args.steps)
     best_fused = min(fused_ms, key=fused_ms.get)
     ms_full_fused = fused_ms[best_fused]

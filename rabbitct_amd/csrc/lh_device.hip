@@ -277,11 +277,13 @@ __device__ __forceinline__ float l2_update(__amdgpu_buffer_rsrc_t rs, uint32_t b
 #endif
 template <int PREC>
 constexpr int kFence = PREC == LH_PREC_F32 ? LH_FENCE32 : 4;
-// updates per scheduling group of the L2-gather fallback, by run length
-// (measured at L=256, where most footprints take it: with R=16 one voxel's
-// loads at a time is fastest, with R=32 four)
+// updates per scheduling group of the L2-gather fallback: four voxels' tap
+// loads in flight per lane (measured at L=256, where most footprints take it:
+// 124 views on 16-voxel runs 3.65 ms with groups of one, 3.27 with two, 3.16
+// with four, 3.43 with eight; 32-voxel runs 3.64 / 3.31 / 4.24 ms with two /
+// four / eight)
 template <int R>
-constexpr int kFenceL2 = R == 32 ? 4 : 1;
+constexpr int kFenceL2 = 4;
 // voxels per parking segment of a run (the line ranges' alignment)
 template <int R>
 constexpr int kSeg = R < 16 ? R : 16;

@@ -129,7 +129,7 @@ class SparseViewReconstructor:
     fallback."""
 
     def __init__(self, weights: str | None = DEFAULT_WEIGHTS, threshold: float | None = None,
-                 net: unet.UNet | None = None, sparse_kernel=("fused", 16, 32, True),
+                 net: unet.UNet | None = None, sparse_kernel=("fused", 32, 16, True),
                  full_kernel=("plain", 32, 16, True)):
         self.net = net or unet.UNet(seed=0)
         self.meta = {}
