@@ -309,6 +309,39 @@ __device__ __forceinline__ void tail16_mfma(f32x4 (&acc)[MT][NT], const uint16_t
   }
 }
 
+// tail16_mfma with the 15 K chunks' weight operands held in registers
+// (W[5 dz + j]), loaded once by a persistent kernel for all its tiles: no
+// weight load is issued between a tile's prefetch of the next halo and its
+// MFMAs, so waiting for a weight operand never waits for that prefetch.  The
+// same MFMAs in the same order as tail16_mfma.
+template <int NT, int MT, int NW>
+__device__ __forceinline__ void tail16_mfma_reg(f32x4 (&acc)[MT][NT], const uint16_t *lds,
+    const bf16x8 (&W)[NW][NT], int w, int m, int kg)
+{
+  constexpr int RS16 = kHX * kHY * 16;
+  const int half = kg >> 1, cofs = 8 * (kg & 1);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int e  = 2 * j + half;
+    const int ee = e >= 9 ? 8 : e;   // the pad: zero weights, as in tail16_mfma
+    const uint16_t *base = lds + ((w + ee / 3) * kHX + m + ee % 3) * 16 + cofs;
+#pragma unroll
+    for (int p = 0; p < MT + 2; ++p) {
+      const bf16x8 A =
+          __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + p * RS16));
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const int r = p - dz;
+        if (r >= 0 && r < MT)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, W[5 * dz + j][t], acc[r][t],
+                0, 0, 0);
+      }
+    }
+  }
+}
+
 // Epilogue of a convolution tile: bias, ReLU, bfloat16 into an LDS transpose
 // buffer (this wave's rows: [MT][16 voxels][COUT]), then 16-byte stores of 8
 // channels; the fused pooling and 1x1x1 head read the same buffer.  Begins
@@ -669,6 +702,17 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
   int k = local, t = xcd * perXcd + k;
   if (k >= perXcd || t >= nTiles)
     return;
+  // one output tile (NT = 1): the weight operands of every tile in 60
+  // registers, loaded before the first halo copy (two workgroups per CU,
+  // which the halo buffers' LDS sets, leave room for them; NT = 2 would
+  // halve that occupancy and loads them per tile)
+  constexpr bool kRegW = NT == 1;
+  bf16x8 W[kRegW ? 15 : 1][NT];
+  if constexpr (kRegW) {
+#pragma unroll
+    for (int c = 0; c < 15; ++c)
+      load_b(W[c], wrow, wtile, c);
+  }
   int n, x0, y0, z0;
   coords(t, n, x0, y0, z0);
   stage16_dma<MT>(a, lds, n, x0, y0, z0);
@@ -691,7 +735,10 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt)
         acc[r][tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    tail16_mfma<NT, MT>(acc, lds + buf * BUF, wrow, wtile, 0, w, m, kg);
+    if constexpr (kRegW)
+      tail16_mfma_reg<NT, MT, 15>(acc, lds + buf * BUF, W, w, m, kg);
+    else
+      tail16_mfma<NT, MT>(acc, lds + buf * BUF, wrow, wtile, 0, w, m, kg);
     conv_epilogue<NT, MT>(a, acc, lds + buf * BUF, n, x0, y0, z0, w, lane, m, kg);
     if (!next)
       break;
