@@ -808,7 +808,7 @@ __global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict
     uint16_t av[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      av[j] = (8 * kg + j < 27) ? halo[hoff[j] + r * kHX * kHY] : (uint16_t)0;
+      av[j] = halo[hoff[j] + r * kHX * kHY];   // pad taps read tap 0 (zero weights)
     const uint4 a4 = make_uint4((uint32_t)av[0] | ((uint32_t)av[1] << 16),
         (uint32_t)av[2] | ((uint32_t)av[3] << 16), (uint32_t)av[4] | ((uint32_t)av[5] << 16),
         (uint32_t)av[6] | ((uint32_t)av[7] << 16));
@@ -838,8 +838,9 @@ __global__ __launch_bounds__(256) void conv_first_kernel(const float *__restrict
 // reads (zeros outside the volume: enc1b's padding), so enc1a's 32 bytes
 // per voxel never go to HBM and come back 2.1 times as enc1b's halo.  Each
 // MFMA has the operands of conv_first_kernel (one halo voxel per A row, its
-// 27 taps then 5 zeros) and of the 16-channel path, and the roundings are
-// theirs: the outputs are the two-kernel form's bit for bit.
+// 27 taps then 5 copies of tap 0 against zero weights) and of the
+// 16-channel path, and the roundings are theirs: the outputs are the
+// two-kernel form's bit for bit.
 __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restrict__ in,
     const uint16_t *__restrict__ w0, const float *__restrict__ b0, ConvLdsArgs a)
 {
@@ -848,12 +849,11 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
   constexpr int IV = IX * IY * IZ;
   constexpr int HV = kHX * kHY * (MT + 2);                // enc1b's halo voxels
   constexpr int GROUPS = (HV + 15) / 16;                  // MFMAs of enc1a
+  static_assert(GROUPS % 4 == 0, "enc1a's groups split evenly over the four waves");
   __shared__ uint16_t inh[IV];
-  // input-halo index of tap (0, 0, 0) of each halo voxel (the last group's
-  // pad entries repeat the last voxel)
-  __shared__ uint16_t vbase[GROUPS * 16];
-  // enc1b's halo [MT+2][kHY][kHX][16], then the epilogue's transpose buffer
-  __shared__ __align__(16) uint16_t halo[HV * 16];
+  // enc1b's halo [MT+2][kHY][kHX][16] (and the last enc1a group's pad
+  // voxels, never read), then the epilogue's transpose buffer
+  __shared__ __align__(16) uint16_t halo[GROUPS * 16 * 16];
   const int D = a.D;
   const int txN = D / 16, tyN = D / 4, tzN = D / MT;
   int tile = blockIdx.x;
@@ -879,10 +879,6 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
         iv[k] = in[(((size_t)n * D + gz) * D + gy) * D + gx];
     }
   }
-  for (int i = threadIdx.x; i < GROUPS * 16; i += 256) {
-    const int v = min(i, HV - 1);
-    vbase[i] = (uint16_t)(((v / (kHX * kHY)) * IY + (v / kHX) % kHY) * IX + v % kHX);
-  }
 #pragma unroll
   for (int k = 0; k < ITER; ++k) {
     const int i = threadIdx.x + 256 * k;
@@ -894,7 +890,10 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
   const int m = lane & 15, kg = lane >> 4;
   {
     // enc1a: halo voxels 16 g .. 16 g + 15 per MFMA (lane m's A row is voxel
-    // 16 g + m, its taps 8 kg .. 8 kg + 7); wave w takes groups w, w + 4, ...
+    // 16 g + m, its taps 8 kg .. 8 kg + 7; the pad voxels of the last group
+    // repeat the last voxel, and the pad taps 27..31 read tap 0, whose
+    // products meet zero weights); wave w takes groups w, w + 4, ..., all
+    // unrolled so that the gathers of later groups overlap earlier MFMAs
     const bf16x8 B =
         __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(w0 + m * 32 + 8 * kg));
     int toff[8];
@@ -905,12 +904,15 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
       toff[j] = ((tp / 9) * IY + (tp / 3) % 3) * IX + tp % 3;
     }
     const float bias = b0[m];
-    for (int g = w; g < GROUPS; g += 4) {
-      const int base = vbase[16 * g + m];
+#pragma unroll
+    for (int gi = 0; gi < GROUPS / 4; ++gi) {
+      const int g    = w + 4 * gi;
+      const int v    = min(16 * g + m, HV - 1);
+      const int base = ((v / (kHX * kHY)) * IY + (v / kHX) % kHY) * IX + v % kHX;
       uint16_t av[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        av[j] = (8 * kg + j < 27) ? inh[base + toff[j]] : (uint16_t)0;
+        av[j] = inh[base + toff[j]];
       const uint4 a4 = make_uint4((uint32_t)av[0] | ((uint32_t)av[1] << 16),
           (uint32_t)av[2] | ((uint32_t)av[3] << 16), (uint32_t)av[4] | ((uint32_t)av[5] << 16),
           (uint32_t)av[6] | ((uint32_t)av[7] << 16));
@@ -919,8 +921,7 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
       uint16_t *hp = halo + (16 * g + 4 * kg) * 16 + m;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (16 * g + 4 * kg + i < HV)
-          hp[16 * i] = f2bf(fmaxf(acc[i] + bias, 0.0f));
+        hp[16 * i] = f2bf(fmaxf(acc[i] + bias, 0.0f));
     }
   }
   // halo voxels outside the volume hold enc1b's zero padding, not enc1a's
