@@ -110,6 +110,11 @@ constexpr int kHX = 18, kHY = 6;
 #ifndef LH_STAGE_BATCH
 #define LH_STAGE_BATCH 8
 #endif
+// 1: passes whose channels come from one tensor are staged by direct-to-LDS
+// buffer loads
+#ifndef LH_STAGE_DMA
+#define LH_STAGE_DMA 1
+#endif
 constexpr int kStageBatch = LH_STAGE_BATCH;
 
 template <int CC, int MT>
@@ -135,6 +140,34 @@ __device__ __forceinline__ void stage_halo(const ConvLdsArgs &a, uint16_t *lds, 
     const uint32_t c0 = (uint32_t)(sr.ofs + (first ? cbase : cbase - a.c0));
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+#if LH_STAGE_DMA
+    // direct-to-LDS: unit i (voxel i / Q, channels 8 (i % Q) ..) lands 16
+    // bytes at lds + 8 i, every unit of the thread in flight at once and no
+    // registers held; units outside the volume read past the buffer range
+    // and land zeros.  Returns once this wave's copies have landed.
+    const int wbase = threadIdx.x & ~63;
+#pragma unroll 1
+    for (int k = 0; k < ITER; ++k) {
+      const int i = threadIdx.x + k * 256;
+      if (i < TOTAL) {
+        const int q = i % Q, v = i / Q;
+        const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        const bool in = (unsigned)gx < (unsigned)D && (unsigned)gy < (unsigned)D &&
+                        (unsigned)gz < (unsigned)D;
+        const int sx = half ? (gx >> 1) : gx, sy = half ? (gy >> 1) : gy,
+                  sz = half ? (gz >> 1) : gz;
+        const uint32_t off =
+            in ? ((((uint32_t)n * Ds + sz) * Ds + sy) * Ds + sx) * ld * 2u + (c0 + 8 * q) * 2u
+               : kMissOffset;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(lds + 8 * (wbase + k * 256)), 16,
+            (int)off, 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    return;
+#endif
 #pragma unroll 1
     for (int b0 = 0; b0 < ITER; b0 += kStageBatch) {
       u32x4 val[kStageBatch];
@@ -221,6 +254,9 @@ __device__ __forceinline__ void stage_coarse(const ConvLdsArgs &a, uint16_t *lds
   const ConvSrc &sr = a.s0;
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
+#if LH_STAGE_DMA
+  const int wbase = threadIdx.x & ~63;   // as in stage_halo: unit i lands at lds + 8 i
+#endif
   u32x4 val[ITER];
 #pragma unroll
   for (int k = 0; k < ITER; ++k) {
@@ -234,9 +270,19 @@ __device__ __forceinline__ void stage_coarse(const ConvLdsArgs &a, uint16_t *lds
       const uint32_t off = in ? (uint32_t)(((((uint32_t)n * Dc + cz) * Dc + cy) * Dc + cx) *
                                            (uint32_t)sr.ld + sr.ofs + cbase + 8 * q) * 2u
                               : kMissOffset;
+#if LH_STAGE_DMA
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+          (__attribute__((address_space(3))) void *)(lds + 8 * (wbase + k * 256)), 16,
+          (int)off, 0, 0, 0);
+#else
       val[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+#endif
     }
   }
+#if LH_STAGE_DMA
+  __builtin_amdgcn_s_waitcnt(0);
+  return;
+#endif
 #pragma unroll
   for (int k = 0; k < ITER; ++k) {
     const int i = threadIdx.x + k * 256;
