@@ -682,12 +682,30 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
 // at lds + 8 i, 1 KB per wave instruction; units outside the volume read past
 // the buffer range and land zeros) into one of two LDS buffers while the
 // previous tile computes and runs its epilogue in the other.
+template <int MT> constexpr int stage16_iter() { return (kHX * kHY * (MT + 2) * 2 + 255) / 256; }
+
+// Byte offsets of this thread's halo units from the halo's first voxel
+// (channel offset included), the same for every tile of a launch.
+template <int MT>
+__device__ __forceinline__ void stage16_rel(const ConvLdsArgs &a, uint32_t (&rel)[stage16_iter<MT>()])
+{
+  constexpr int TOTAL = kHX * kHY * (MT + 2) * 2;
+  const uint32_t D = (uint32_t)a.D, ld = (uint32_t)a.s0.ld, c0 = (uint32_t)a.s0.ofs;
+#pragma unroll
+  for (int k = 0; k < stage16_iter<MT>(); ++k) {
+    const int i = threadIdx.x + k * 256;
+    const int q = i & 1, v = i >> 1;
+    const uint32_t hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+    rel[k] = i < TOTAL ? (((hz * D + hy) * D + hx) * ld + c0 + 8 * q) * 2u : 0u;
+  }
+}
+
 template <int MT>
 __device__ __forceinline__ void stage16_dma(const ConvLdsArgs &a, uint16_t *buf, int n, int x0,
-    int y0, int z0)
+    int y0, int z0, const uint32_t (&rel)[stage16_iter<MT>()])
 {
   constexpr int TOTAL = kHX * kHY * (MT + 2) * 2;  // 16-byte units
-  constexpr int ITER  = (TOTAL + 255) / 256;
+  constexpr int ITER  = stage16_iter<MT>();
   const int D         = a.D;
   const ConvSrc &sr   = a.s0;                      // channels [0, c0) = all 16
   const uint32_t ld   = (uint32_t)sr.ld;
@@ -695,6 +713,21 @@ __device__ __forceinline__ void stage16_dma(const ConvLdsArgs &a, uint16_t *buf,
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void *)sr.p, (short)0, sr.bytes, kRsrcWord3);
   const int wbase = threadIdx.x & ~63;
+  // a halo inside the volume (most tiles): the tile's base offset plus the
+  // thread's precomputed unit offsets
+  if (x0 > 0 && y0 > 0 && z0 > 0 && x0 + 16 < D && y0 + 4 < D && z0 + MT < D) {
+    const uint32_t base =
+        ((((uint32_t)n * D + z0 - 1) * D + y0 - 1) * D + x0 - 1) * ld * 2u;
+#pragma unroll
+    for (int k = 0; k < ITER; ++k) {
+      const int i = threadIdx.x + k * 256;
+      if (i < TOTAL)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+            (__attribute__((address_space(3))) void *)(buf + 8 * (wbase + k * 256)), 16,
+            (int)(base + rel[k]), 0, 0, 0);
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < ITER; ++k) {
     const int i = threadIdx.x + k * 256;
@@ -759,9 +792,11 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
     for (int c = 0; c < 15; ++c)
       load_b(W[c], wrow, wtile, c);
   }
+  uint32_t rel[stage16_iter<MT>()];
+  stage16_rel<MT>(a, rel);
   int n, x0, y0, z0;
   coords(t, n, x0, y0, z0);
-  stage16_dma<MT>(a, lds, n, x0, y0, z0);
+  stage16_dma<MT>(a, lds, n, x0, y0, z0, rel);
   int buf = 0;
   for (;;) {
     // this tile's halo has landed, and every wave is done with the other
@@ -773,7 +808,7 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
     int n2 = 0, x2 = 0, y2 = 0, z2 = 0;
     if (next) {
       coords(t2, n2, x2, y2, z2);
-      stage16_dma<MT>(a, lds + (buf ^ 1) * BUF, n2, x2, y2, z2);
+      stage16_dma<MT>(a, lds + (buf ^ 1) * BUF, n2, x2, y2, z2, rel);
     }
     f32x4 acc[MT][NT];
 #pragma unroll
