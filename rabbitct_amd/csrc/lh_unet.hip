@@ -392,9 +392,48 @@ __device__ __forceinline__ void tail16_mfma_reg(f32x4 (&acc)[MT][NT], const uint
 // buffer (this wave's rows: [MT][16 voxels][COUT]), then 16-byte stores of 8
 // channels; the fused pooling and 1x1x1 head read the same buffer.  Begins
 // with a barrier: every wave's reads of the tile's input from lds are done.
+// The epilogue's own loads, which a persistent kernel issues before its
+// prefetch of the next tile (loads return in order: one issued after the
+// prefetch would make its use wait for it); the one-tile kernels load them
+// in the epilogue (PRE = false).  bias[t]: this lane's output
+// channel 16 t + m; hin[k]: the fused head's input at this wave's voxel
+// 64 k + lane of the tile.
+template <int MT> constexpr int head_per_lane() { return (MT * 16 + 63) / 64; }
+
+template <int NT>
+__device__ __forceinline__ void load_bias(const ConvLdsArgs &a, int m, float (&bias)[NT])
+{
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+    bias[t] = a.bias ? a.bias[16 * t + m] : 0.0f;
+}
+
 template <int NT, int MT>
+__device__ __forceinline__ void load_hin(const ConvLdsArgs &a, int n, int x0, int y0, int z0,
+    int w, int lane, float (&hin)[head_per_lane<MT>()])
+{
+#pragma unroll
+  for (int k = 0; k < head_per_lane<MT>(); ++k)
+    hin[k] = 0.0f;
+  if constexpr (NT == 1) {
+    if (a.hout) {
+      const int D = a.D;
+#pragma unroll
+      for (int k = 0; k < head_per_lane<MT>(); ++k) {
+        const int vv = 64 * k + lane;
+        if ((MT * 16) % 64 == 0 || vv < MT * 16) {
+          const size_t vo = (((size_t)n * D + z0 + vv / 16) * D + y0 + w) * D + x0 + vv % 16;
+          hin[k] = a.hin[vo];
+        }
+      }
+    }
+  }
+}
+
+template <int NT, int MT, bool PRE = false>
 __device__ __forceinline__ void conv_epilogue(const ConvLdsArgs &a, f32x4 (&acc)[MT][NT],
-    uint16_t *lds, int n, int x0, int y0, int z0, int w, int lane, int m, int kg)
+    uint16_t *lds, int n, int x0, int y0, int z0, int w, int lane, int m, int kg,
+    const float (&biasv)[NT], const float (&hin)[head_per_lane<MT>()])
 {
   const int D = a.D;
   constexpr int COUT = NT * 16;
@@ -403,7 +442,7 @@ __device__ __forceinline__ void conv_epilogue(const ConvLdsArgs &a, f32x4 (&acc)
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int co = 16 * t + m;
-    const float bias = a.bias ? a.bias[co] : 0.0f;
+    const float bias = PRE ? biasv[t] : (a.bias ? a.bias[co] : 0.0f);
 #pragma unroll
     for (int r = 0; r < MT; ++r)
 #pragma unroll
@@ -484,7 +523,7 @@ __device__ __forceinline__ void conv_epilogue(const ConvLdsArgs &a, f32x4 (&acc)
           const int vx = vv % 16, r = vv / 16;
           const uint4 *hv = reinterpret_cast<const uint4 *>(ob + (r * 16 + vx) * 16);
           const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + vx;
-          float sum = a.hin[vo] + a.hb[0];
+          float sum = (PRE ? hin[v0 / 64] : a.hin[vo]) + a.hb[0];
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
             const uint4 hq = hv[q];
@@ -672,7 +711,8 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       }
     }
   }
-  conv_epilogue<NT, MT>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg);
+  const float biasv[NT] = {}, hin[head_per_lane<MT>()] = {};   // loaded in the epilogue
+  conv_epilogue<NT, MT>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
 }
 
 // Persistent form of the 16-input-channel convolution (one 16-channel pass;
@@ -792,6 +832,8 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
     for (int c = 0; c < 15; ++c)
       load_b(W[c], wrow, wtile, c);
   }
+  float biasv[NT];
+  load_bias<NT>(a, m, biasv);
   uint32_t rel[stage16_iter<MT>()];
   stage16_rel<MT>(a, rel);
   int n, x0, y0, z0;
@@ -803,6 +845,9 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
     // buffer (the previous tile's epilogue)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    // the fused head's inputs of this tile, loaded before the prefetch
+    float hin[head_per_lane<MT>()];
+    load_hin<NT, MT>(a, n, x0, y0, z0, w, lane, hin);
     const int k2 = k + stride, t2 = xcd * perXcd + k2;
     const bool next = k2 < perXcd && t2 < nTiles;
     int n2 = 0, x2 = 0, y2 = 0, z2 = 0;
@@ -820,7 +865,8 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
       tail16_mfma_reg<NT, MT, 15>(acc, lds + buf * BUF, W, w, m, kg);
     else
       tail16_mfma<NT, MT>(acc, lds + buf * BUF, wrow, wtile, 0, w, m, kg);
-    conv_epilogue<NT, MT>(a, acc, lds + buf * BUF, n, x0, y0, z0, w, lane, m, kg);
+    conv_epilogue<NT, MT, true>(a, acc, lds + buf * BUF, n, x0, y0, z0, w, lane, m, kg, biasv,
+        hin);
     if (!next)
       break;
     k = k2;
@@ -1027,7 +1073,8 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
   for (int r = 0; r < MT; ++r)
     acc[r][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   tail16_mfma<1, MT>(acc, halo, wrow, (size_t)16 * a.kp, 0, w, m, kg);
-  conv_epilogue<1, MT>(a, acc, halo, n, x0, y0, z0, w, lane, m, kg);
+  const float biasv[1] = {}, hin[head_per_lane<MT>()] = {};    // loaded in the epilogue
+  conv_epilogue<1, MT>(a, acc, halo, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
 }
 
 // Weight gradient dW[co][tap][ci] += sum_v g[v][co] * x[v + off(tap)][ci].
