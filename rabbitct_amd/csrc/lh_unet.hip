@@ -564,6 +564,14 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
   const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
   const size_t wtile = (size_t)16 * a.kp;
   const int F = a.cin / 32;
+  // with the upsampled channels staged coarse (dec1a: four workgroups per
+  // CU, two short staging phases) the epilogue's bias is loaded ahead of the
+  // staging, so that its latency is not exposed after the last MFMA
+  // (measured 2-7% faster there; the other forms load it in the epilogue and
+  // keep their register counts)
+  float biasv[NT] = {};
+  if constexpr (UPC)
+    load_bias<NT>(a, m, biasv);
   f32x4 acc[MT][NT];
 #pragma unroll
   for (int r = 0; r < MT; ++r)
@@ -711,8 +719,10 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       }
     }
   }
-  const float biasv[NT] = {}, hin[head_per_lane<MT>()] = {};   // loaded in the epilogue
-  conv_epilogue<NT, MT>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
+  float hin[head_per_lane<MT>()] = {};
+  if constexpr (UPC)
+    load_hin<NT, MT>(a, n, x0, y0, z0, w, lane, hin);
+  conv_epilogue<NT, MT, UPC>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
 }
 
 // Persistent form of the 16-input-channel convolution (one 16-channel pass;
