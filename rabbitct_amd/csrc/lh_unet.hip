@@ -313,16 +313,25 @@ template <int NT, int MT> constexpr size_t conv_lds_bytes()
 // k0 + 5 dz + j: lanes kg < 2 hold tap (dz, 2j), kg >= 2 tap (dz, 2j+1), each
 // lane 8 of the 16 channels), input planes outer: the A operand of halo plane
 // p at (dy, dx) serves output rows p - dz for dz = 0, 1, 2.
-template <int NT, int MT>
+// PRE: the first chunks' weight operands (k0 + 5 dz) come loaded in B0.
+template <int NT, int MT, bool PRE = false>
 __device__ __forceinline__ void tail16_mfma(f32x4 (&acc)[MT][NT], const uint16_t *lds,
-    const uint16_t *wrow, size_t wtile, int k0, int w, int m, int kg)
+    const uint16_t *wrow, size_t wtile, int k0, int w, int m, int kg,
+    const bf16x8 (&B0)[PRE ? 3 : 1][NT])
 {
   constexpr int RS16 = kHX * kHY * 16;
   const int half = kg >> 1, cofs = 8 * (kg & 1);
   bf16x8 Bn[3][NT];
 #pragma unroll
   for (int dz = 0; dz < 3; ++dz)
-    load_b(Bn[dz], wrow, wtile, k0 + 5 * dz);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      if constexpr (PRE)
+        Bn[dz][t] = B0[dz][t];
+  if constexpr (!PRE)
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+      load_b(Bn[dz], wrow, wtile, k0 + 5 * dz);
   for (int j = 0; j < 5; ++j) {
     bf16x8 B[3][NT];
 #pragma unroll
@@ -582,10 +591,16 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
   for (int pass = 0; pass < F; ++pass) {
     if (pass)
       __syncthreads();
-    // UPC: every 32-channel pass holds upsampled channels, staged coarse
-    if constexpr (UPC)
+    // UPC: every 32-channel pass holds upsampled channels, staged coarse;
+    // the pass's first weight operands are loaded before the staging, whose
+    // wait then covers them
+    bf16x8 Bw[UPC ? 3 : 1][NT];
+    if constexpr (UPC) {
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz)
+        load_b(Bw[dz], wrow, wtile, (9 * dz) * F + pass);
       stage_coarse<MT>(a, lds, 32 * pass, n, x0, y0, z0);
-    else
+    } else
       stage_halo<32, MT>(a, lds, 32 * pass, n, x0, y0, z0);
     __syncthreads();
     if constexpr (UPC) {
@@ -596,7 +611,9 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
       bf16x8 Bn[3][NT];
 #pragma unroll
       for (int dz = 0; dz < 3; ++dz)
-        load_b(Bn[dz], wrow, wtile, (9 * dz) * F + pass);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          Bn[dz][t] = Bw[dz][t];
       for (int dyx = 0; dyx < 9; ++dyx) {
         bf16x8 B[3][NT];
 #pragma unroll
@@ -686,13 +703,19 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
   if constexpr (TAIL) {
     if (F)
       __syncthreads();
+    // UPC: the tail's first weight operands loaded before its staging, as above
+    bf16x8 Bt[UPC ? 3 : 1][NT];
+    if constexpr (UPC)
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz)
+        load_b(Bt[dz], wrow, wtile, 27 * F + 5 * dz);
     stage_halo<16, MT>(a, lds, 32 * F, n, x0, y0, z0);
     __syncthreads();
     // chunk 27F + 5 dz + j: lanes kg < 2 hold tap (dz, 2j), kg >= 2 tap
     // (dz, 2j+1) (e = 9 is the zero pad); each lane reads 8 of the 16 channels
     const int half = kg >> 1, cofs = 8 * (kg & 1);
     if constexpr (NT <= 2) {
-      tail16_mfma<NT, MT>(acc, lds, wrow, wtile, 27 * F, w, m, kg);
+      tail16_mfma<NT, MT, UPC>(acc, lds, wrow, wtile, 27 * F, w, m, kg, Bt);
     } else {
       bf16x8 Bn[NT];
       load_b(Bn, wrow, wtile, 27 * F);
@@ -874,7 +897,7 @@ __global__ __launch_bounds__(256) void conv16_persist_kernel(ConvLdsArgs a)
     if constexpr (kRegW)
       tail16_mfma_reg<NT, MT, 15>(acc, lds + buf * BUF, W, w, m, kg);
     else
-      tail16_mfma<NT, MT>(acc, lds + buf * BUF, wrow, wtile, 0, w, m, kg);
+      tail16_mfma<NT, MT>(acc, lds + buf * BUF, wrow, wtile, 0, w, m, kg, W);
     conv_epilogue<NT, MT, true>(a, acc, lds + buf * BUF, n, x0, y0, z0, w, lane, m, kg, biasv,
         hin);
     if (!next)
@@ -1082,7 +1105,8 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
 #pragma unroll
   for (int r = 0; r < MT; ++r)
     acc[r][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  tail16_mfma<1, MT>(acc, halo, wrow, (size_t)16 * a.kp, 0, w, m, kg);
+  const bf16x8 none[1][1] = {};
+  tail16_mfma<1, MT>(acc, halo, wrow, (size_t)16 * a.kp, 0, w, m, kg, none);
   const float biasv[1] = {}, hin[head_per_lane<MT>()] = {};    // loaded in the epilogue
   conv_epilogue<1, MT>(a, acc, halo, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
 }
