@@ -1025,6 +1025,17 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
   rest /= tyN;
   const int tz = rest % tzN, n = rest / tzN;
   const int x0 = tx * 16, y0 = ty * 4, z0 = tz * MT;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  // enc1b's bias and first weight operands, loaded ahead of the input so
+  // that their latency is covered by the input's
+  const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
+  float biasv[1];
+  load_bias<1>(a, m, biasv);
+  bf16x8 Bt[3][1];
+#pragma unroll
+  for (int dz = 0; dz < 3; ++dz)
+    load_b(Bt[dz], wrow, (size_t)16 * a.kp, 5 * dz);
   // the input around the halo, all of a thread's loads issued before the
   // first is used; zeros outside the volume
   constexpr int ITER = (IV + 255) / 256;
@@ -1046,8 +1057,6 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
       inh[i] = f2bf(iv[k]);
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int m = lane & 15, kg = lane >> 4;
   {
     // enc1a: halo voxels 16 g .. 16 g + 15 per MFMA (lane m's A row is voxel
     // 16 g + m, its taps 8 kg .. 8 kg + 7; the pad voxels of the last group
@@ -1100,15 +1109,14 @@ __global__ __launch_bounds__(256) void conv_first16_kernel(const float *__restri
   __syncthreads();
   // enc1b: the 16-channel path on the computed halo, then bias, ReLU, the
   // stores and the fused pool
-  const uint16_t *wrow = a.w + (size_t)m * a.kp + 8 * kg;
   f32x4 acc[MT][1];
 #pragma unroll
   for (int r = 0; r < MT; ++r)
     acc[r][0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const bf16x8 none[1][1] = {};
-  tail16_mfma<1, MT>(acc, halo, wrow, (size_t)16 * a.kp, 0, w, m, kg, none);
-  const float biasv[1] = {}, hin[head_per_lane<MT>()] = {};    // loaded in the epilogue
-  conv_epilogue<1, MT>(a, acc, halo, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
+  tail16_mfma<1, MT, true>(acc, halo, wrow, (size_t)16 * a.kp, 0, w, m, kg, Bt);
+  float hin[head_per_lane<MT>()];
+  load_hin<1, MT>(a, n, x0, y0, z0, w, lane, hin);
+  conv_epilogue<1, MT, true>(a, acc, halo, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
 }
 
 // Weight gradient dW[co][tap][ci] += sum_v g[v][co] * x[v + off(tap)][ci].
