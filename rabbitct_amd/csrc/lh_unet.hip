@@ -550,6 +550,103 @@ __device__ __forceinline__ void conv_epilogue(const ConvLdsArgs &a, f32x4 (&acc)
   }
 }
 
+// Epilogue of the wide-output forms (NT > 2), whose MFMAs take the weights
+// as the A operand and the activations as the B operand: lane (m, kg) of
+// wave w holds acc[r][t][i] = output voxel (x0 + m, y0 + w, z0 + r),
+// channel 16 t + 4 kg + i, four consecutive channels (8 bytes in bfloat16),
+// stored from registers without the LDS transpose, which costs these forms
+// NT x MT x 4 two-byte LDS writes per lane.  The fused pooling reads the
+// same [MT][16 voxels][COUT] LDS buffer as in conv_epilogue (8-byte writes).
+template <int NT, int MT>
+__device__ __forceinline__ void conv_epilogue_wide(const ConvLdsArgs &a, f32x4 (&acc)[MT][NT],
+    uint16_t *lds, int n, int x0, int y0, int z0, int w, int m, int kg)
+{
+  const int D = a.D;
+  constexpr int COUT = NT * 16;
+  uint2 h[MT][NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    float bias[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      bias[i] = a.bias ? a.bias[16 * t + 4 * kg + i] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < MT; ++r) {
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v[i] = acc[r][t][i] + bias[i];
+        if (a.relu)
+          v[i] = fmaxf(v[i], 0.0f);
+      }
+      h[r][t] = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                           (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+    }
+  }
+  if (a.y) {
+#pragma unroll
+    for (int r = 0; r < MT; ++r) {
+      const size_t vo = (((size_t)n * D + z0 + r) * D + y0 + w) * D + x0 + m;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        uint2 *dst = reinterpret_cast<uint2 *>(a.y + vo * a.ldy + a.yofs + 16 * t + 4 * kg);
+        uint2 val = h[r][t];
+        if (a.accumulate) {
+          const uint2 old = *dst;
+          const uint32_t nv[2] = { val.x, val.y };
+          const uint32_t ov[2] = { old.x, old.y };
+          uint32_t rv[2];
+#pragma unroll
+          for (int k = 0; k < 2; ++k)
+            rv[k] = (uint32_t)f2bf(__uint_as_float(nv[k] << 16) + __uint_as_float(ov[k] << 16)) |
+                    ((uint32_t)f2bf(__uint_as_float(nv[k] & 0xffff0000u) +
+                                    __uint_as_float(ov[k] & 0xffff0000u)) << 16);
+          val = make_uint2(rv[0], rv[1]);
+        }
+        *dst = val;
+      }
+    }
+  }
+  if (a.pool) {
+    // every wave is done reading the tile's input from lds
+    __syncthreads();
+    uint16_t *ob = lds + (size_t)w * MT * 16 * COUT;
+#pragma unroll
+    for (int r = 0; r < MT; ++r)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        *reinterpret_cast<uint2 *>(ob + (r * 16 + m) * COUT + 16 * t + 4 * kg) = h[r][t];
+    __syncthreads();
+    constexpr int C8 = COUT / 8;
+    constexpr int CELLS = 8 * 2 * (MT / 2);
+    const int Dh = D >> 1;
+    for (int u = threadIdx.x; u < CELLS * C8; u += 256) {
+      const int c8 = u % C8, cell = u / C8;
+      const int cx = cell % 8, cy = (cell / 8) % 2, cz = cell / 16;
+      float sum[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int zz = 2 * cz + (q >> 2), yy = 2 * cy + ((q >> 1) & 1), xx = 2 * cx + (q & 1);
+        const uint4 v = *reinterpret_cast<const uint4 *>(lds + (size_t)yy * MT * 16 * COUT +
+                                                        (zz * 16 + xx) * COUT + 8 * c8);
+        const uint32_t w4[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          sum[2 * k] += __uint_as_float(w4[k] << 16);
+          sum[2 * k + 1] += __uint_as_float(w4[k] & 0xffff0000u);
+        }
+      }
+      uint32_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        o[k] = (uint32_t)f2bf(0.125f * sum[2 * k]) | ((uint32_t)f2bf(0.125f * sum[2 * k + 1]) << 16);
+      const size_t vo = ((((size_t)n * Dh + (z0 >> 1) + cz) * Dh + (y0 >> 1) + cy) * Dh) +
+                        (x0 >> 1) + cx;
+      *reinterpret_cast<uint4 *>(a.pool + vo * a.ldp + a.pofs + 8 * c8) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
+}
+
 // Input channels are staged in passes: F = cin/32 passes of 32 channels
 // (one tap per K chunk), then with TAIL a pass of the last 16 channels (tap
 // pairs per K chunk), matching the weight packing.
@@ -695,7 +792,7 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
           const bf16x8 A = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + r * RS32));
 #pragma unroll
           for (int t = 0; t < NT; ++t)
-            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t], A, acc[r][t], 0, 0, 0);
         }
       }
     }
@@ -736,7 +833,7 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
                 __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(base + r * RS16));
 #pragma unroll
             for (int t = 0; t < NT; ++t)
-              acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[r][t], 0, 0, 0);
+              acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t], A, acc[r][t], 0, 0, 0);
           }
         }
       }
@@ -745,7 +842,10 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(ConvLdsArgs a)
   float hin[head_per_lane<MT>()] = {};
   if constexpr (UPC)
     load_hin<NT, MT>(a, n, x0, y0, z0, w, lane, hin);
-  conv_epilogue<NT, MT, UPC>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
+  if constexpr (NT > 2)
+    conv_epilogue_wide<NT, MT>(a, acc, lds, n, x0, y0, z0, w, m, kg);
+  else
+    conv_epilogue<NT, MT, UPC>(a, acc, lds, n, x0, y0, z0, w, lane, m, kg, biasv, hin);
 }
 
 // Persistent form of the 16-input-channel convolution (one 16-channel pass;
