@@ -16,7 +16,9 @@
 //   share one chunk: 5 chunks per dz (the tenth tap of each is zero).
 // Lane l holds A rows (voxel l%16) and B columns (channel l%16) for the 8
 // K entries 8*(l/16)..+7 of a chunk: 8 consecutive channels of one tap, one
-// 16-byte load each.  The backward-data pass is the same kernel with the taps
+// 16-byte load each.  The wide-output forms (more than 32 output channels)
+// pass the same two operands in the other order, so that their results
+// come out voxel by voxel with four consecutive channels per lane.  The backward-data pass is the same kernel with the taps
 // mirrored and input/output channels swapped in the weight packing.  The
 // weight gradient is a separate MFMA kernel that reduces over voxels (K)
 // from transposed rows staged in LDS.
